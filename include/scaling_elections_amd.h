@@ -1,0 +1,141 @@
+/*
+ * scaling_elections_amd.h -- C ABI of the MI355X (gfx950) Schulze engine.
+ *
+ * Plain C: pointers, sizes and status codes; no C++ exceptions, no torch or numpy types.
+ * Every entry point returns SE_OK (0) or a negative SE_ERR_* code; the message of the
+ * last failure on the calling thread is available from se_last_error().  Messages that
+ * the reference raises as Python RuntimeError keep the reference's exact text.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository root):
+ *   se_closure / se_closure_cpu   compute_strongest_paths(preferences, *, allow_tma,
+ *                                 allow_gpu, tile_size)      scaling_elections.cu:863-989
+ *   se_count / se_count_rankings  build_pairwise_preferences + populate_preferences_from_ranking
+ *                                                            scaling_elections.py:37-88
+ *   se_rank                       get_winner_and_ranking     scaling_elections.py:274-296
+ *   se_device_count / se_device_info   log_gpus()            scaling_elections.cu:996-1016
+ *   se_reduce_f32                 reduce(float32[])          scaling_elections.cu:1019-1032
+ *
+ * Threading: calls on one device are serialised by a per-device lock; the library owns
+ * its device scratch (a grow-only cache per device) and one stream per device.  The
+ * *_device variants take caller-owned device pointers and an optional hipStream_t
+ * (passed as void*, NULL = the library's stream for the current device).
+ */
+#ifndef SCALING_ELECTIONS_AMD_H
+#define SCALING_ELECTIONS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------------- */
+#define SE_OK 0
+#define SE_ERR_INVALID (-1)      /* bad argument (shape, size, pointer, flag)            */
+#define SE_ERR_TILE (-2)         /* "Unsupported tile size" and the CPU tile checks      */
+#define SE_ERR_ALLOC (-3)        /* "Failed to allocate memory on device"                */
+#define SE_ERR_DEVICE (-4)       /* kernel launch / execution error                      */
+#define SE_ERR_COPY (-5)         /* host<->device copy failed                            */
+#define SE_ERR_NO_DEVICE (-6)    /* no usable HIP device                                 */
+#define SE_ERR_INTERRUPTED (-7)  /* "Stopped by signal"                                  */
+#define SE_ERR_RANGE (-8)        /* value does not fit the requested width (u16)         */
+
+/* ---- closure flags ------------------------------------------------------------------ */
+#define SE_FLAG_AUTO_WIDTH 0u       /* u16 packed when max(p0) <= 65535, else u32         */
+#define SE_FLAG_FORCE_U32 1u        /* always run the 32-bit closure                      */
+#define SE_FLAG_FORCE_U16 2u        /* require the 16-bit closure (SE_ERR_RANGE if not)   */
+#define SE_FLAG_TIME_KERNELS 4u     /* record hipEvents around every kernel (se_timing)   */
+#define SE_FLAG_ALLOW_TMA 8u        /* accepted and ignored: AMD GPUs have no TMA         */
+                                    /* (scaling_elections.cu:555-558)                     */
+
+/* Per-call measurements, filled when a non-NULL pointer is passed. */
+typedef struct se_timing {
+    double total_ms;          /* whole device pipeline: p0 init + rounds + output      */
+    double init_ms;           /* p0 init kernels (SE_FLAG_TIME_KERNELS)                */
+    double diag_ms;           /* sum over rounds: pivot-tile kernel                    */
+    double panel_ms;          /* sum over rounds: row/column panel kernel              */
+    double indep_ms;          /* sum over rounds: independent-tile kernel              */
+    double output_ms;         /* widening + diagonal zeroing + unpadding               */
+    uint32_t rounds;          /* pivot rounds run                                      */
+    uint32_t pivot;           /* pivot depth used                                      */
+    uint32_t width_bits;      /* 16 or 32                                              */
+    uint32_t padded_n;        /* matrix side after zero padding                        */
+    uint64_t indep_cells;     /* cell updates issued by the independent kernel (all    */
+                              /* rounds) = sum over rounds of (N - T)^2 * T            */
+} se_timing;
+
+/* ---- diagnostics -------------------------------------------------------------------- */
+const char *se_last_error(void);
+int se_version(void);
+int se_device_count(int *count);
+/* name: caller buffer of name_len bytes (NUL-terminated on return). */
+int se_device_info(int device, char *name, size_t name_len, int *compute_units,
+                   size_t *global_mem_bytes, int *major, int *minor, char *arch, size_t arch_len);
+
+/* ---- strongest paths (the closure) ---------------------------------------------------- */
+/* Host buffers.  prefs: n x n uint32, row stride `row_stride` elements (>= n).  paths:
+ * n x n uint32, contiguous, written entirely (p0 rule, max-min closure, zero diagonal).
+ * tile_size must be 4, 8, 16 or 32 (scaling_elections.cu:895-901; 0 is rejected as in
+ * the reference); the result does not depend on it.  Any n >= 0 is accepted: the GPU
+ * matrix is zero padded, which is exact. */
+int se_closure(const uint32_t *prefs, uint32_t n, size_t row_stride, uint32_t *paths,
+               uint32_t tile_size, uint32_t flags, se_timing *timing);
+
+/* Device buffers (caller-owned), same semantics; paths has row stride paths_stride. */
+int se_closure_device(const uint32_t *prefs_dev, uint32_t n, size_t row_stride,
+                      uint32_t *paths_dev, size_t paths_stride, uint32_t flags, void *stream,
+                      se_timing *timing);
+
+/* CPU branch of compute_strongest_paths (allow_gpu=False), scaling_elections.cu:933-988:
+ * tile_size 0 picks 4 when n >= 4, else "Number of candidates should be at least 4,
+ * ideally divisible by 4"; tile_size > n gives "Tile size should be less than or equal
+ * to the number of candidates"; tiles other than 4..128 powers of two give "Unsupported
+ * tile size".  threads <= 0 uses every hardware thread. */
+int se_closure_cpu(const uint32_t *prefs, uint32_t n, size_t row_stride, uint32_t *paths,
+                   uint32_t tile_size, int threads);
+
+/* Serial CPU closure (the CLI's validation baseline), scaling_elections.py:91-125. */
+int se_closure_cpu_serial(const uint32_t *prefs, uint32_t n, size_t row_stride, uint32_t *paths);
+
+/* ---- pairwise preference counter ---------------------------------------------------- */
+/* positions: m x n int32 (row v = voter v), positions[v][c] = place of candidate c in
+ * voter v's ballot.  Every row must hold n distinct values (a strict order), which the
+ * builder guarantees by filling incomplete ballots.  prefs: n x n uint32 (row stride
+ * prefs_stride) receives d[i][j] = #{v : positions[v][i] < positions[v][j]}. */
+int se_count(const int32_t *positions, uint32_t m, uint32_t n, uint32_t *prefs,
+             size_t prefs_stride);
+int se_count_device(const int32_t *positions_dev, uint32_t m, uint32_t n, uint32_t *prefs_dev,
+                    size_t prefs_stride, void *stream, double *kernel_ms);
+
+/* General ballots, exactly populate_preferences_from_ranking applied to each ballot
+ * (scaling_elections.py:46-48), duplicates included: ids is the concatenation of m
+ * (already filled) ballots, offsets has m+1 entries.  Every id must lie in [0, n). */
+int se_count_rankings(const int32_t *ids, const int64_t *offsets, uint32_t m, uint32_t n,
+                      uint32_t *prefs, size_t prefs_stride);
+
+/* ---- winner and ranking ------------------------------------------------------------- */
+/* wins[i] = #{j != i : p[i][j] > p[j][i]}; ranking = indices by descending wins, ties in
+ * ascending index (Python's stable sort), scaling_elections.py:287-292. */
+int se_rank(const uint32_t *paths, uint32_t n, size_t row_stride, uint32_t *wins,
+            uint32_t *ranking);
+int se_rank_device(const uint32_t *paths_dev, uint32_t n, size_t row_stride, uint32_t *wins_host,
+                   uint32_t *ranking_host, void *stream);
+
+/* ---- misc ---------------------------------------------------------------------------- */
+/* Sequential float sum, the reference's non-CUDA reduce() (std::accumulate order). */
+float se_reduce_f32(const float *data, size_t count);
+
+/* Interruption: se_interrupt() sets a flag that running closures check between pivot
+ * rounds ("Stopped by signal", scaling_elections.cu:758-759); se_install_sigint() makes
+ * SIGINT set it (returns the previous handler state for se_restore_sigint). */
+void se_interrupt(void);
+void se_clear_interrupt(void);
+int se_install_sigint(void);
+void se_restore_sigint(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SCALING_ELECTIONS_AMD_H */

@@ -1,0 +1,423 @@
+// api.hip -- the extern "C" boundary: error state, per-device contexts and the entry
+// points declared in include/scaling_elections_amd.h.  No C++ exception crosses it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <csignal>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "se_internal.h"
+
+namespace se {
+
+// Implemented in closure.hip / count.hip / rank.hip / cpu_closure.cpp.
+int closure_device_impl(DeviceContext* ctx, const uint32_t* d, uint32_t n, size_t ld, uint32_t* out,
+                        size_t ldo, uint32_t flags, hipStream_t s, se_timing* timing);
+int count_device_impl(DeviceContext* ctx, const int32_t* pos, uint32_t m, uint32_t n, uint32_t* d,
+                      size_t ldd, hipStream_t s, double* kernel_ms);
+int count_rankings_device_impl(DeviceContext* ctx, const int32_t* ids, const int64_t* offsets,
+                               uint32_t m, uint32_t n, uint32_t* d, size_t ldd, hipStream_t s);
+int wins_device_impl(DeviceContext* ctx, const uint32_t* p, uint32_t n, size_t ld, uint32_t* wins,
+                     hipStream_t s);
+int closure_cpu_impl(const uint32_t* d, uint32_t n, size_t ld, uint32_t* p, uint32_t tile, int threads);
+int closure_cpu_serial_impl(const uint32_t* d, uint32_t n, size_t ld, uint32_t* p);
+
+namespace {
+thread_local std::string t_last_error;
+std::mutex g_ctx_mutex;
+std::unique_ptr<DeviceContext> g_ctx[64];
+struct sigaction g_old_sigint;
+std::mutex g_sig_mutex;
+int g_sigint_depth = 0;
+}  // namespace
+
+std::atomic<int> g_interrupted{0};
+
+void set_error(const std::string& msg) { t_last_error = msg; }
+void clear_error() { t_last_error.clear(); }
+
+DeviceContext* current_context() {
+    int dev = -1;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        set_error("No HIP devices available");
+        return nullptr;
+    }
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        set_error("Failed to get current device");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> g(g_ctx_mutex);
+    if (!g_ctx[dev]) {
+        auto ctx = std::make_unique<DeviceContext>();
+        ctx->device = dev;
+        if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+            set_error("Failed to create a HIP stream");
+            return nullptr;
+        }
+        g_ctx[dev] = std::move(ctx);
+    }
+    return g_ctx[dev].get();
+}
+
+void* scratch(DeviceContext* ctx, Slot s, size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    if (ctx->cap[s] >= bytes) return ctx->ptr[s];
+    if (ctx->ptr[s]) {
+        // the previous contents are dead; make sure no queued kernel still uses them
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipFree(ctx->ptr[s]);
+        ctx->ptr[s] = nullptr;
+        ctx->cap[s] = 0;
+    }
+    void* p = nullptr;
+    size_t want = round_up(bytes, (size_t)1 << 20);
+    if (hipMalloc(&p, want) != hipSuccess || p == nullptr) {
+        (void)hipGetLastError();
+        set_error("Failed to allocate memory on device");
+        return nullptr;
+    }
+    ctx->ptr[s] = p;
+    ctx->cap[s] = want;
+    return p;
+}
+
+static hipStream_t pick_stream(DeviceContext* ctx, void* stream) {
+    return stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+}
+
+}  // namespace se
+
+using namespace se;
+
+static void sigint_handler(int) { g_interrupted.store(1); }
+
+extern "C" {
+
+const char* se_last_error(void) { return t_last_error.c_str(); }
+
+int se_version(void) { return 10000; }
+
+int se_device_count(int* count) {
+    if (!count) {
+        set_error("count must not be NULL");
+        return SE_ERR_INVALID;
+    }
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *count = 0;
+        set_error("No HIP devices available");
+        return SE_ERR_NO_DEVICE;
+    }
+    *count = c;
+    return SE_OK;
+}
+
+int se_device_info(int device, char* name, size_t name_len, int* compute_units, size_t* global_mem_bytes,
+                   int* major, int* minor, char* arch, size_t arch_len) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("Failed to get device properties");
+        return SE_ERR_NO_DEVICE;
+    }
+    if (name && name_len) {
+        std::strncpy(name, prop.name, name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    if (arch && arch_len) {
+        std::strncpy(arch, prop.gcnArchName, arch_len - 1);
+        arch[arch_len - 1] = 0;
+    }
+    if (compute_units) *compute_units = prop.multiProcessorCount;
+    if (global_mem_bytes) *global_mem_bytes = prop.totalGlobalMem;
+    if (major) *major = prop.major;
+    if (minor) *minor = prop.minor;
+    return SE_OK;
+}
+
+static bool gpu_tile_ok(uint32_t tile) { return tile == 4 || tile == 8 || tile == 16 || tile == 32; }
+
+int se_closure(const uint32_t* prefs, uint32_t n, size_t row_stride, uint32_t* paths, uint32_t tile_size,
+               uint32_t flags, se_timing* timing) {
+    clear_error();
+    if (!gpu_tile_ok(tile_size)) {
+        set_error("Unsupported tile size");
+        return SE_ERR_TILE;
+    }
+    if (n > 0 && (!prefs || !paths || row_stride < n)) {
+        set_error("invalid preferences/paths buffers");
+        return SE_ERR_INVALID;
+    }
+    g_interrupted.store(0);
+    DeviceContext* ctx = current_context();
+    if (!ctx) return SE_ERR_NO_DEVICE;
+    std::lock_guard<std::mutex> g(ctx->lock);
+    if (n == 0) {
+        if (timing) *timing = se_timing{};
+        return SE_OK;
+    }
+    const size_t bytes = (size_t)n * n * sizeof(uint32_t);
+    uint32_t* d = static_cast<uint32_t*>(scratch(ctx, kSlotPrefs, bytes));
+    if (!d) return SE_ERR_ALLOC;
+    hipStream_t s = ctx->stream;
+    if (!hip_ok(hipMemcpy2DAsync(d, n * sizeof(uint32_t), prefs, row_stride * sizeof(uint32_t),
+                                 n * sizeof(uint32_t), n, hipMemcpyHostToDevice, s),
+                "Failed to copy data from host to device"))
+        return SE_ERR_COPY;
+    // the staged input is dead after p0 init, so the output reuses its buffer
+    int rc = closure_device_impl(ctx, d, n, n, d, n, flags, s, timing);
+    if (rc != SE_OK) {
+        (void)hipStreamSynchronize(s);
+        return rc;
+    }
+    if (!hip_ok(hipStreamSynchronize(s), "CUDA operations did not complete successfully")) return SE_ERR_DEVICE;
+    if (!hip_ok(hipMemcpyAsync(paths, d, bytes, hipMemcpyDeviceToHost, s), "Failed to copy data from device to host"))
+        return SE_ERR_COPY;
+    if (!hip_ok(hipStreamSynchronize(s), "CUDA transfers did not complete successfully")) return SE_ERR_COPY;
+    return SE_OK;
+}
+
+int se_closure_device(const uint32_t* prefs_dev, uint32_t n, size_t row_stride, uint32_t* paths_dev,
+                      size_t paths_stride, uint32_t flags, void* stream, se_timing* timing) {
+    clear_error();
+    if (n > 0 && (!prefs_dev || !paths_dev || row_stride < n || paths_stride < n)) {
+        set_error("invalid preferences/paths device buffers");
+        return SE_ERR_INVALID;
+    }
+    g_interrupted.store(0);
+    DeviceContext* ctx = current_context();
+    if (!ctx) return SE_ERR_NO_DEVICE;
+    std::lock_guard<std::mutex> g(ctx->lock);
+    return closure_device_impl(ctx, prefs_dev, n, row_stride, paths_dev, paths_stride, flags,
+                               pick_stream(ctx, stream), timing);
+}
+
+int se_closure_cpu(const uint32_t* prefs, uint32_t n, size_t row_stride, uint32_t* paths, uint32_t tile_size,
+                   int threads) {
+    clear_error();
+    if (n > 0 && (!prefs || !paths || row_stride < n)) {
+        set_error("invalid preferences/paths buffers");
+        return SE_ERR_INVALID;
+    }
+    // tile resolution, scaling_elections.cu:943-980
+    static const uint32_t kTiles[] = {4, 8, 16, 32, 64, 128};
+    if (tile_size == 0) {
+        for (uint32_t t : kTiles)
+            if (n >= t) {
+                tile_size = t;
+                break;
+            }
+        if (tile_size == 0) {
+            set_error("Number of candidates should be at least 4, ideally divisible by 4");
+            return SE_ERR_TILE;
+        }
+    } else {
+        if (tile_size > n) {
+            set_error("Tile size should be less than or equal to the number of candidates");
+            return SE_ERR_TILE;
+        }
+        if (std::find(std::begin(kTiles), std::end(kTiles), tile_size) == std::end(kTiles)) {
+            set_error("Unsupported tile size");
+            return SE_ERR_TILE;
+        }
+    }
+    g_interrupted.store(0);
+    return closure_cpu_impl(prefs, n, row_stride, paths, tile_size, threads);
+}
+
+int se_closure_cpu_serial(const uint32_t* prefs, uint32_t n, size_t row_stride, uint32_t* paths) {
+    clear_error();
+    if (n > 0 && (!prefs || !paths || row_stride < n)) {
+        set_error("invalid preferences/paths buffers");
+        return SE_ERR_INVALID;
+    }
+    g_interrupted.store(0);
+    return closure_cpu_serial_impl(prefs, n, row_stride, paths);
+}
+
+int se_count(const int32_t* positions, uint32_t m, uint32_t n, uint32_t* prefs, size_t prefs_stride) {
+    clear_error();
+    if (n == 0) return SE_OK;
+    if (!prefs || prefs_stride < n || (m > 0 && !positions)) {
+        set_error("invalid positions/preferences buffers");
+        return SE_ERR_INVALID;
+    }
+    DeviceContext* ctx = current_context();
+    if (!ctx) return SE_ERR_NO_DEVICE;
+    std::lock_guard<std::mutex> g(ctx->lock);
+    hipStream_t s = ctx->stream;
+    const size_t pbytes = (size_t)m * n * sizeof(int32_t);
+    const size_t dbytes = (size_t)n * n * sizeof(uint32_t);
+    int32_t* pos = static_cast<int32_t*>(scratch(ctx, kSlotAux, pbytes));
+    if (!pos) return SE_ERR_ALLOC;
+    uint32_t* d = static_cast<uint32_t*>(scratch(ctx, kSlotPrefs, dbytes));
+    if (!d) return SE_ERR_ALLOC;
+    if (m > 0 && !hip_ok(hipMemcpyAsync(pos, positions, pbytes, hipMemcpyHostToDevice, s),
+                         "Failed to copy data from host to device"))
+        return SE_ERR_COPY;
+    int rc = count_device_impl(ctx, pos, m, n, d, n, s, nullptr);
+    if (rc != SE_OK) return rc;
+    if (!hip_ok(hipMemcpy2DAsync(prefs, prefs_stride * sizeof(uint32_t), d, n * sizeof(uint32_t),
+                                 n * sizeof(uint32_t), n, hipMemcpyDeviceToHost, s),
+                "Failed to copy data from device to host"))
+        return SE_ERR_COPY;
+    if (!hip_ok(hipStreamSynchronize(s), "CUDA transfers did not complete successfully")) return SE_ERR_COPY;
+    return SE_OK;
+}
+
+int se_count_device(const int32_t* positions_dev, uint32_t m, uint32_t n, uint32_t* prefs_dev,
+                    size_t prefs_stride, void* stream, double* kernel_ms) {
+    clear_error();
+    if (n == 0) return SE_OK;
+    if (!prefs_dev || prefs_stride < n || (m > 0 && !positions_dev)) {
+        set_error("invalid positions/preferences device buffers");
+        return SE_ERR_INVALID;
+    }
+    DeviceContext* ctx = current_context();
+    if (!ctx) return SE_ERR_NO_DEVICE;
+    std::lock_guard<std::mutex> g(ctx->lock);
+    return count_device_impl(ctx, positions_dev, m, n, prefs_dev, prefs_stride, pick_stream(ctx, stream),
+                             kernel_ms);
+}
+
+int se_count_rankings(const int32_t* ids, const int64_t* offsets, uint32_t m, uint32_t n, uint32_t* prefs,
+                      size_t prefs_stride) {
+    clear_error();
+    if (n == 0) return SE_OK;
+    if (!prefs || prefs_stride < n || !offsets || (m > 0 && offsets[m] > 0 && !ids)) {
+        set_error("invalid ballots/preferences buffers");
+        return SE_ERR_INVALID;
+    }
+    const int64_t total = offsets[m];
+    for (uint32_t v = 0; v < m; ++v)
+        if (offsets[v] > offsets[v + 1] || offsets[v] < 0) {
+            set_error("ballot offsets must be non-decreasing");
+            return SE_ERR_INVALID;
+        }
+    for (int64_t i = 0; i < total; ++i)
+        if (ids[i] < 0 || (uint32_t)ids[i] >= n) {
+            set_error("candidate ids must lie in [0, n)");
+            return SE_ERR_INVALID;
+        }
+    DeviceContext* ctx = current_context();
+    if (!ctx) return SE_ERR_NO_DEVICE;
+    std::lock_guard<std::mutex> g(ctx->lock);
+    hipStream_t s = ctx->stream;
+    const size_t ibytes = (size_t)std::max<int64_t>(total, 1) * sizeof(int32_t);
+    const size_t obytes = (size_t)(m + 1) * sizeof(int64_t);
+    char* aux = static_cast<char*>(scratch(ctx, kSlotAux, round_up(ibytes, 256) + obytes));
+    if (!aux) return SE_ERR_ALLOC;
+    int32_t* ids_dev = reinterpret_cast<int32_t*>(aux);
+    int64_t* off_dev = reinterpret_cast<int64_t*>(aux + round_up(ibytes, 256));
+    uint32_t* d = static_cast<uint32_t*>(scratch(ctx, kSlotPrefs, (size_t)n * n * sizeof(uint32_t)));
+    if (!d) return SE_ERR_ALLOC;
+    if ((total > 0 && !hip_ok(hipMemcpyAsync(ids_dev, ids, (size_t)total * sizeof(int32_t), hipMemcpyHostToDevice, s),
+                              "Failed to copy data from host to device")) ||
+        !hip_ok(hipMemcpyAsync(off_dev, offsets, obytes, hipMemcpyHostToDevice, s),
+                "Failed to copy data from host to device"))
+        return SE_ERR_COPY;
+    int rc = count_rankings_device_impl(ctx, ids_dev, off_dev, m, n, d, n, s);
+    if (rc != SE_OK) return rc;
+    if (!hip_ok(hipMemcpy2DAsync(prefs, prefs_stride * sizeof(uint32_t), d, n * sizeof(uint32_t),
+                                 n * sizeof(uint32_t), n, hipMemcpyDeviceToHost, s),
+                "Failed to copy data from device to host"))
+        return SE_ERR_COPY;
+    if (!hip_ok(hipStreamSynchronize(s), "CUDA transfers did not complete successfully")) return SE_ERR_COPY;
+    return SE_OK;
+}
+
+static void stable_rank(const uint32_t* wins, uint32_t n, uint32_t* ranking) {
+    std::iota(ranking, ranking + n, 0u);
+    std::stable_sort(ranking, ranking + n, [wins](uint32_t a, uint32_t b) { return wins[a] > wins[b]; });
+}
+
+int se_rank_device(const uint32_t* paths_dev, uint32_t n, size_t row_stride, uint32_t* wins_host,
+                   uint32_t* ranking_host, void* stream) {
+    clear_error();
+    if (n == 0) return SE_OK;
+    if (!paths_dev || row_stride < n || !wins_host) {
+        set_error("invalid paths/wins buffers");
+        return SE_ERR_INVALID;
+    }
+    DeviceContext* ctx = current_context();
+    if (!ctx) return SE_ERR_NO_DEVICE;
+    std::lock_guard<std::mutex> g(ctx->lock);
+    hipStream_t s = pick_stream(ctx, stream);
+    uint32_t* w = static_cast<uint32_t*>(scratch(ctx, kSlotSmall, (size_t)n * sizeof(uint32_t)));
+    if (!w) return SE_ERR_ALLOC;
+    int rc = wins_device_impl(ctx, paths_dev, n, row_stride, w, s);
+    if (rc != SE_OK) return rc;
+    if (!hip_ok(hipMemcpyAsync(wins_host, w, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost, s),
+                "Failed to copy data from device to host") ||
+        !hip_ok(hipStreamSynchronize(s), "CUDA transfers did not complete successfully"))
+        return SE_ERR_COPY;
+    if (ranking_host) stable_rank(wins_host, n, ranking_host);
+    return SE_OK;
+}
+
+int se_rank(const uint32_t* paths, uint32_t n, size_t row_stride, uint32_t* wins, uint32_t* ranking) {
+    clear_error();
+    if (n == 0) return SE_OK;
+    if (!paths || row_stride < n || !wins) {
+        set_error("invalid paths/wins buffers");
+        return SE_ERR_INVALID;
+    }
+    DeviceContext* ctx = current_context();
+    if (!ctx) return SE_ERR_NO_DEVICE;
+    std::lock_guard<std::mutex> g(ctx->lock);
+    hipStream_t s = ctx->stream;
+    uint32_t* p = static_cast<uint32_t*>(scratch(ctx, kSlotPaths, (size_t)n * n * sizeof(uint32_t)));
+    uint32_t* w = static_cast<uint32_t*>(scratch(ctx, kSlotSmall, (size_t)n * sizeof(uint32_t)));
+    if (!p || !w) return SE_ERR_ALLOC;
+    if (!hip_ok(hipMemcpy2DAsync(p, n * sizeof(uint32_t), paths, row_stride * sizeof(uint32_t),
+                                 n * sizeof(uint32_t), n, hipMemcpyHostToDevice, s),
+                "Failed to copy data from host to device"))
+        return SE_ERR_COPY;
+    int rc = wins_device_impl(ctx, p, n, n, w, s);
+    if (rc != SE_OK) return rc;
+    if (!hip_ok(hipMemcpyAsync(wins, w, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost, s),
+                "Failed to copy data from device to host") ||
+        !hip_ok(hipStreamSynchronize(s), "CUDA transfers did not complete successfully"))
+        return SE_ERR_COPY;
+    if (ranking) stable_rank(wins, n, ranking);
+    return SE_OK;
+}
+
+float se_reduce_f32(const float* data, size_t count) {
+    float acc = 0.0f;
+    for (size_t i = 0; i < count; ++i) acc = acc + data[i];
+    return acc;
+}
+
+void se_interrupt(void) { g_interrupted.store(1); }
+void se_clear_interrupt(void) { g_interrupted.store(0); }
+
+// Reference-counted so that concurrent callers restore the original handler exactly once.
+int se_install_sigint(void) {
+    std::lock_guard<std::mutex> g(g_sig_mutex);
+    if (g_sigint_depth++ > 0) return SE_OK;
+    struct sigaction sa;
+    std::memset(&sa, 0, sizeof(sa));
+    sa.sa_handler = sigint_handler;
+    sigemptyset(&sa.sa_mask);
+    if (sigaction(SIGINT, &sa, &g_old_sigint) != 0) {
+        --g_sigint_depth;
+        return SE_ERR_INVALID;
+    }
+    return SE_OK;
+}
+
+void se_restore_sigint(void) {
+    std::lock_guard<std::mutex> g(g_sig_mutex);
+    if (g_sigint_depth > 0 && --g_sigint_depth == 0) (void)sigaction(SIGINT, &g_old_sigint, nullptr);
+}
+
+}  // extern "C"

@@ -1,0 +1,71 @@
+// se_internal.h -- shared host-side plumbing of the HIP library: error state, per-device
+// context (stream, lock, grow-only scratch cache) and launch helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+#include <string>
+
+#include "../../include/scaling_elections_amd.h"
+
+namespace se {
+
+// Thread-local message of the last failing call (returned by se_last_error()).
+void set_error(const std::string& msg);
+void clear_error();
+
+// Interruption flag shared by the CPU and GPU drivers.
+extern std::atomic<int> g_interrupted;
+
+// Scratch slots of a device context.  Each slot is a grow-only allocation reused by
+// later calls on the same device.
+enum Slot : int {
+    kSlotWork = 0,     // padded closure matrix (u16 or u32)
+    kSlotPrefs = 1,    // staged input matrix (host entry points)
+    kSlotPaths = 2,    // staged output matrix (host entry points)
+    kSlotSmall = 3,    // scalars: max(p0), counters
+    kSlotPivot = 4,    // pivot-tile hand-off between kernels
+    kSlotAux = 5,      // positions / wins staging
+    kSlotCount = 8
+};
+
+struct DeviceContext {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    std::mutex lock;
+    void* ptr[kSlotCount] = {};
+    size_t cap[kSlotCount] = {};
+};
+
+// Context of the current HIP device (created on first use).  Returns nullptr and sets
+// the error when no device is usable.
+DeviceContext* current_context();
+
+// Ensures slot `s` holds at least `bytes`; returns the device pointer or nullptr (error
+// set to "Failed to allocate memory on device").
+void* scratch(DeviceContext* ctx, Slot s, size_t bytes);
+
+inline bool hip_ok(hipError_t e, const char* what) {
+    if (e == hipSuccess) return true;
+    std::string msg = what;
+    if (msg.empty()) msg = hipGetErrorString(e);
+    set_error(msg);
+    return false;
+}
+
+inline size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
+
+}  // namespace se
+
+#define SE_CHECK_LAUNCH(what)                                      \
+    do {                                                           \
+        hipError_t _e = hipGetLastError();                         \
+        if (_e != hipSuccess) {                                    \
+            se::set_error(std::string(what) + ": " + hipGetErrorString(_e)); \
+            return SE_ERR_DEVICE;                                  \
+        }                                                          \
+    } while (0)

@@ -1,0 +1,192 @@
+"""GPU parity: the HIP closure, counter and ranking against the CPU oracle, bit-exact.
+
+Sizes here finish on the oracle in seconds; the 32768-candidate workload is covered by
+test_gpu_large.py (hash fixture + independent per-row Dijkstra checks)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from common import load_golden, schulze45_ballots
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu(d, tile=32, tma=False):
+    from scalingelections_amd import compute_strongest_paths
+
+    return compute_strongest_paths(d, allow_gpu=True, tile_size=tile, allow_tma=tma)
+
+
+def test_golden_fixtures_gpu(gpu_lib):
+    for case in load_golden("paradox.json")["cases"]:
+        d = np.array(case["d"], dtype=np.uint32)
+        for t in (4, 8, 16, 32):
+            assert _gpu(d, t).tolist() == case["p"]
+    _, g = schulze45_ballots()
+    assert _gpu(np.array(g["d"], dtype=np.uint32)).tolist() == g["p"]
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 17, 31, 32, 33, 100, 127, 128, 129, 255, 256, 257, 300, 513, 777, 1000])
+def test_closure_random_sizes(gpu_lib, oracle_mod, n):
+    print(f"closure parity n={n}", flush=True)
+    rng = np.random.default_rng(1000 + n)
+    d = rng.integers(0, max(2, 2 * n), (n, n)).astype(np.uint32)
+    want = oracle_mod.closure_tiled(d, 16)
+    got = _gpu(d)
+    assert got.dtype == np.uint32 and got.shape == (n, n)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("tile", [4, 8, 16, 32])
+def test_tile_size_and_tma_do_not_change_result(gpu_lib, oracle_mod, tile):
+    rng = np.random.default_rng(tile)
+    d = rng.integers(0, 500, (200, 200)).astype(np.uint32)
+    want = oracle_mod.closure_serial(d)
+    assert np.array_equal(_gpu(d, tile), want)
+    assert np.array_equal(_gpu(d, tile, tma=True), want)
+
+
+def _timed(d, flags):
+    from scalingelections_amd import strongest_paths_timed
+
+    return strongest_paths_timed(d, flags)
+
+
+@pytest.mark.parametrize("hi,width", [(1000, 16), (65536, 16), (65537, 32), (2**32 - 1, 32)])
+def test_width_selection_is_exact(gpu_lib, oracle_mod, hi, width):
+    """u16 when max(p0) <= 65535 (boundary included), u32 above; forced widths agree."""
+    from scalingelections_amd import _lib
+
+    rng = np.random.default_rng(hi % 1000)
+    n = 300
+    d = rng.integers(0, hi, (n, n), dtype=np.uint64).astype(np.uint32)
+    if hi == 65536:
+        d[3, 5], d[5, 3] = 65535, 0  # max(p0) hits the boundary exactly
+    if hi == 65537:
+        d[3, 5], d[5, 3] = 65536, 0
+    want = oracle_mod.closure_tiled(d, 16)
+    got, t = _timed(d, 0)
+    assert t["width_bits"] == width
+    assert np.array_equal(got, want)
+    got32, t32 = _timed(d, _lib.SE_FLAG_FORCE_U32)
+    assert t32["width_bits"] == 32 and np.array_equal(got32, want)
+    if width == 16:
+        got16, t16 = _timed(d, _lib.SE_FLAG_FORCE_U16)
+        assert t16["width_bits"] == 16 and np.array_equal(got16, want)
+    else:
+        with pytest.raises(RuntimeError, match="exceed 65535"):
+            _timed(d, _lib.SE_FLAG_FORCE_U16)
+
+
+def test_degenerate_inputs(gpu_lib, oracle_mod):
+    for d in (np.zeros((64, 64), np.uint32), np.full((70, 70), 7, np.uint32),
+              np.triu(np.full((130, 130), 9, np.uint32), 1), np.eye(40, dtype=np.uint32) * 5):
+        assert np.array_equal(_gpu(d), oracle_mod.closure_serial(d))
+    assert _gpu(np.zeros((0, 0), np.uint32)).shape == (0, 0)
+
+
+def test_repeat_calls_reuse_scratch(gpu_lib, oracle_mod):
+    rng = np.random.default_rng(77)
+    for n in (600, 100, 600, 1200, 64):
+        d = rng.integers(0, 90000, (n, n)).astype(np.uint32)
+        assert np.array_equal(_gpu(d), oracle_mod.closure_tiled(d, 16))
+
+
+def test_device_entry_point_with_caller_buffers(gpu_lib, oracle_mod):
+    """se_closure_device on caller-owned device memory, padded output stride, caller stream."""
+    from scalingelections_amd import _lib, devmem
+
+    rng = np.random.default_rng(5)
+    n = 333
+    d = rng.integers(0, 4000, (n, n)).astype(np.uint32)
+    want = oracle_mod.closure_serial(d)
+    dev = devmem.DeviceBuffer.from_array(d)
+    out = devmem.DeviceBuffer(n * (n + 3) * 4)
+    s = devmem.Stream()
+    t = _lib.se_timing()
+    rc = _lib.lib().se_closure_device(dev.ptr, n, n, out.ptr, n + 3, _lib.SE_FLAG_TIME_KERNELS,
+                                      ctypes.c_void_p(s.handle), ctypes.byref(t))
+    _lib.check(rc)
+    s.synchronize()
+    host = out.download(np.empty((n, n + 3), dtype=np.uint32))
+    assert np.array_equal(host[:, :n], want)
+    assert t.total_ms > 0 and t.indep_ms > 0 and t.rounds == (n + 31) // 32
+    # in place: output over the input buffer
+    rc = _lib.lib().se_closure_device(dev.ptr, n, n, dev.ptr, n, 0, None, None)
+    _lib.check(rc)
+    devmem.device_synchronize()
+    assert np.array_equal(dev.download(np.empty((n, n), dtype=np.uint32)), want)
+
+
+# ---------------------------------------------------------------- counter
+def test_counter_c1_pipeline(gpu_lib, oracle_mod):
+    """128 candidates x 2000 voters (seed 1): counter, closure, ranking all exact."""
+    from scalingelections_amd import build_pairwise_preferences, get_winner_and_ranking, synth
+
+    R = synth.impartial_culture(2000, 128, seed=1)
+    ballots = list(R)
+    d = build_pairwise_preferences(ballots)
+    assert np.array_equal(d, oracle_mod.build_prefs(ballots))
+    p = _gpu(d)
+    want_p = oracle_mod.closure_serial(d)
+    assert np.array_equal(p, want_p)
+    cands = list(range(128))
+    winner, ranked = get_winner_and_ranking(cands, p)
+    w2, r2 = oracle_mod.winner_and_ranking(cands, want_p)
+    assert (winner, ranked) == (w2, r2)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_counter_incomplete_and_duplicate_ballots(gpu_lib, oracle_mod, seed):
+    from scalingelections_amd import build_pairwise_preferences
+
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(2, 300))
+    m = int(rng.integers(1, 200))
+    ballots = [rng.permutation(n)[: int(rng.integers(1, n + 1))] for _ in range(m)]
+    if seed % 2 == 1:
+        ballots[0] = np.array([0, 1, 0, 1])  # repeated ids take the exact scatter path
+    d = build_pairwise_preferences(ballots)
+    assert np.array_equal(d, oracle_mod.build_prefs(ballots))
+
+
+@pytest.mark.parametrize("m,n", [(1, 1), (0, 5), (3, 129), (257, 64), (1000, 300), (33, 1000)])
+def test_counter_shapes(gpu_lib, oracle_mod, m, n):
+    from scalingelections_amd import count_positions, synth
+
+    R = synth.impartial_culture(m, n, seed=m + n) if m else np.zeros((0, n), np.int32)
+    pos = synth.positions_from_rankings(R) if m else np.zeros((0, n), np.int32)
+    d = count_positions(pos)
+    assert np.array_equal(d, oracle_mod.build_prefs(list(R)) if m else np.zeros((n, n), np.uint32))
+
+
+# ---------------------------------------------------------------- ranking
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 200, 1000])
+def test_ranking_matches_oracle(gpu_lib, oracle_mod, n):
+    from scalingelections_amd import get_winner_and_ranking, wins_and_order
+
+    rng = np.random.default_rng(n)
+    d = rng.integers(0, 3, (n, n)).astype(np.uint32)  # heavy ties
+    p = oracle_mod.closure_serial(d)
+    wins, order = wins_and_order(p)
+    assert wins.tolist() == oracle_mod.wins(p).tolist()
+    assert order.tolist() == oracle_mod.ranking(p).tolist()
+    cands = [f"c{i}" for i in range(n)]
+    assert get_winner_and_ranking(cands, p) == oracle_mod.winner_and_ranking(cands, p)
+
+
+def test_ranking_empty_candidates_raises_like_reference(gpu_lib):
+    from scalingelections_amd import get_winner_and_ranking
+
+    with pytest.raises(IndexError):
+        get_winner_and_ranking([], np.zeros((0, 0), np.uint32))
+
+
+def test_log_gpus_reports_gfx950(gpu_lib, capsys):
+    from scalingelections_amd import device_info, log_gpus
+
+    log_gpus()
+    out = capsys.readouterr().out
+    assert "Device 0" in out
+    assert device_info(0)["arch"].startswith("gfx950")

@@ -1,0 +1,227 @@
+"""CPU-only tests of the product: the C ABI library loads and exports every declared
+symbol, argument validation and error texts, the explicit CPU branch (allow_gpu=False)
+and the serial baseline against the oracle, interruption, input conversion, generators.
+No GPU compute is called here."""
+import ctypes
+import os
+import re
+import signal
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from common import load_golden, p0_numpy
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_symbols():
+    syms = set()
+    inc = os.path.join(ROOT, "include")
+    for f in os.listdir(inc):
+        if f.endswith(".h"):
+            text = open(os.path.join(inc, f)).read()
+            syms |= set(re.findall(r"\b(se_[a-z0-9_]+)\s*\(", text))
+    return syms
+
+
+def test_library_exports_every_declared_symbol():
+    from scalingelections_amd import _lib
+
+    L = _lib.lib()
+    declared = _declared_symbols()
+    assert declared, "no declarations found"
+    for s in sorted(declared):
+        assert hasattr(L, s), s
+    assert set(_lib.EXPORTED) <= declared
+    raw = ctypes.CDLL(_lib.LIB_PATH)
+    for s in declared:
+        getattr(raw, s)
+
+
+def test_library_is_gfx950_build():
+    from scalingelections_amd import _lib
+
+    blob = open(_lib.LIB_PATH, "rb").read()
+    assert b"gfx950" in blob
+
+
+def test_shape_and_dtype_errors():
+    from scalingelections_amd import compute_strongest_paths
+
+    with pytest.raises(RuntimeError, match="^Number of dimensions must be two$"):
+        compute_strongest_paths(np.zeros(4, dtype=np.uint32))
+    with pytest.raises(RuntimeError, match="^Preferences matrix must be square$"):
+        compute_strongest_paths(np.zeros((3, 4), dtype=np.uint32))
+    for dt in (np.int64, np.float32, np.int32, np.uint64):
+        with pytest.raises(TypeError):
+            compute_strongest_paths(np.zeros((4, 4), dtype=dt))
+    with pytest.raises(TypeError):
+        compute_strongest_paths([[0, -1], [1, 0]])
+
+
+def test_gpu_tile_validation_precedes_device_use():
+    from scalingelections_amd import compute_strongest_paths
+
+    d = np.zeros((8, 8), dtype=np.uint32)
+    for t in (0, 1, 2, 3, 5, 64, 128, 2**40):
+        with pytest.raises(RuntimeError, match="^Unsupported tile size$"):
+            compute_strongest_paths(d, allow_gpu=True, tile_size=t)
+
+
+def test_cpu_tile_rules():
+    from scalingelections_amd import compute_strongest_paths
+
+    with pytest.raises(RuntimeError, match="^Number of candidates should be at least 4, ideally divisible by 4$"):
+        compute_strongest_paths(np.zeros((3, 3), dtype=np.uint32))
+    with pytest.raises(RuntimeError, match="^Tile size should be less than or equal to the number of candidates$"):
+        compute_strongest_paths(np.zeros((8, 8), dtype=np.uint32), tile_size=16)
+    with pytest.raises(RuntimeError, match="^Unsupported tile size$"):
+        compute_strongest_paths(np.zeros((8, 8), dtype=np.uint32), tile_size=6)
+
+
+@pytest.mark.parametrize("n,tile", [(4, 0), (5, 4), (37, 8), (64, 16), (100, 32), (130, 64), (128, 128)])
+def test_cpu_branch_matches_oracle(oracle_mod, n, tile):
+    from scalingelections_amd import compute_strongest_paths
+
+    rng = np.random.default_rng(n + tile)
+    d = rng.integers(0, 3 * n, (n, n)).astype(np.uint32)
+    want = oracle_mod.closure_serial(d)
+    got = compute_strongest_paths(d, allow_gpu=False, tile_size=tile)
+    assert got.dtype == np.uint32 and got.shape == (n, n)
+    assert np.array_equal(got, want)
+
+
+def test_cpu_branch_golden():
+    from scalingelections_amd import compute_strongest_paths, compute_strongest_paths_serial
+
+    for case in load_golden("paradox.json")["cases"]:
+        d = np.array(case["d"], dtype=np.uint32)
+        assert compute_strongest_paths(d).tolist() == case["p"]
+        assert compute_strongest_paths_serial(d).tolist() == case["p"]
+
+
+def test_serial_baseline_matches_oracle(oracle_mod):
+    from scalingelections_amd import compute_strongest_paths_serial
+
+    rng = np.random.default_rng(1)
+    for n in (1, 2, 7, 64, 150):
+        d = rng.integers(0, 2**32 - 1, (n, n), dtype=np.uint64).astype(np.uint32)
+        assert np.array_equal(compute_strongest_paths_serial(d), oracle_mod.closure_serial(d))
+
+
+def test_input_conversion_and_no_mutation(oracle_mod):
+    from scalingelections_amd import compute_strongest_paths
+
+    rng = np.random.default_rng(2)
+    d16 = rng.integers(0, 50, (20, 20)).astype(np.uint16)
+    want = oracle_mod.closure_serial(d16.astype(np.uint32))
+    assert np.array_equal(compute_strongest_paths(d16, tile_size=4), want)  # safe cast u16 -> u32
+    big = rng.integers(0, 50, (40, 40)).astype(np.uint32)
+    view = big[::2, ::2]  # non-contiguous: copied
+    before = big.copy()
+    assert np.array_equal(compute_strongest_paths(view, tile_size=4), oracle_mod.closure_serial(np.ascontiguousarray(view)))
+    assert np.array_equal(big, before)
+    assert np.array_equal(compute_strongest_paths(d16.astype(np.uint32).tolist(), tile_size=4), want)
+
+
+def test_cpu_branch_stops_on_sigint():
+    """SIGINT during the CPU branch -> RuntimeError("Stopped by signal"); the previous
+    handler is restored afterwards."""
+    from scalingelections_amd import compute_strongest_paths
+
+    d = np.random.default_rng(0).integers(0, 1000, (2048, 2048)).astype(np.uint32)
+    prev = signal.getsignal(signal.SIGINT)
+
+    def fire():
+        time.sleep(0.05)
+        os.kill(os.getpid(), signal.SIGINT)
+
+    th = threading.Thread(target=fire)
+    th.start()
+    t0 = time.time()
+    with pytest.raises(RuntimeError, match="^Stopped by signal$"):
+        compute_strongest_paths(d, tile_size=4)  # 8.6e9 cell updates: seconds, far above 50 ms
+    th.join()
+    assert time.time() - t0 < 5.0
+    assert signal.getsignal(signal.SIGINT) is prev
+
+
+def test_reduce_sequential_float():
+    from scalingelections_amd import reduce
+
+    x = np.random.default_rng(4).standard_normal(1001).astype(np.float32)
+    acc = np.float32(0)
+    for v in x:
+        acc = np.float32(acc + v)
+    assert reduce(x) == float(acc)
+    assert reduce(np.zeros(0, dtype=np.float32)) == 0.0
+
+
+def test_builder_positions_host_logic(oracle_mod):
+    """Ballot -> position conversion reproduces the reference's fill rule (counted here by
+    the oracle from the positions' implied orders, no GPU)."""
+    from scalingelections_amd import builder
+
+    rng = np.random.default_rng(8)
+    for _ in range(20):
+        n = int(rng.integers(1, 12))
+        ballots = [rng.permutation(n)[: int(rng.integers(1, n + 1))] for _ in range(int(rng.integers(1, 9)))]
+        nn = builder.count_candidates(ballots)
+        pos = builder.rankings_to_positions(ballots, nn)
+        assert pos is not None
+        orders = [np.argsort(row, kind="stable") for row in pos]
+        assert np.array_equal(oracle_mod.build_prefs(orders), oracle_mod.build_prefs(ballots))
+    assert builder.rankings_to_positions([np.array([0, 1, 0])], 2) is None
+
+
+def test_synthesized_preferences_shape_and_rules():
+    from scalingelections_amd import synth
+
+    d = synth.synthesized_preferences(300, seed=3)
+    assert d.dtype == np.uint32
+    off = ~np.eye(300, dtype=bool)
+    ties = (d == 32767) & (d.T == 32767)
+    assert (((d.astype(np.int64) + d.T) == 65535) | ties)[off].all()
+    assert 0.03 < ties[off].mean() < 0.07
+    assert np.array_equal(d, synth.synthesized_preferences(300, seed=3, workers=1))
+    assert p0_numpy(d).max() <= 65535
+
+
+def test_impartial_culture_is_permutations():
+    from scalingelections_amd import synth
+
+    R = synth.impartial_culture(50, 30, seed=1)
+    assert (np.sort(R, axis=1) == np.arange(30)).all()
+    pos = synth.positions_from_rankings(R)
+    assert (np.take_along_axis(pos, R.astype(np.intp), axis=1) == np.arange(30)).all()
+
+
+def test_cli_flags_and_defaults():
+    from scalingelections_amd import cli
+
+    a = cli.parse_args([])
+    assert (a.num_voters, a.num_candidates, a.run_cpu, a.run_gpu, a.no_serial) == (0, 256, False, False, False)
+    assert (a.cpu_tile_size, a.gpu_tile_size, a.warmup, a.repeat) == (16, 32, 1, 1)
+    assert cli.format_throughput(3.5e12) == "3.5 Tcells³/s"
+    assert cli.format_time(0.5) == "500 ms" and cli.format_time(2.0) == "2.00 s"
+
+
+def test_drop_in_module_names():
+    import scaling_elections
+
+    for name in ("compute_strongest_paths", "log_gpus", "reduce", "build_pairwise_preferences",
+                 "get_winner_and_ranking"):
+        assert callable(getattr(scaling_elections, name))
+
+
+def test_product_does_not_import_oracle():
+    pkg = os.path.join(ROOT, "scalingelections_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".hip", ".cpp", ".h")):
+                text = open(os.path.join(dirpath, f)).read()
+                assert "import oracle" not in text and "from oracle" not in text, f
+                assert "liboracle" not in text and "libref" not in text, f
