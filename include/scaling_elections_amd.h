@@ -48,6 +48,8 @@ extern "C" {
 #define SE_FLAG_TIME_KERNELS 4u     /* record hipEvents around every kernel (se_timing)   */
 #define SE_FLAG_ALLOW_TMA 8u        /* accepted and ignored: AMD GPUs have no TMA         */
                                     /* (scaling_elections.cu:555-558)                     */
+#define SE_FLAG_PIVOT32 16u         /* force pivot depth 32 (default: 64 for 16-bit, 32    */
+#define SE_FLAG_PIVOT64 32u         /* for 32-bit); the result never depends on it         */
 
 /* Per-call measurements, filled when a non-NULL pointer is passed. */
 typedef struct se_timing {

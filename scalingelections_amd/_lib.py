@@ -28,6 +28,8 @@ SE_FLAG_FORCE_U32 = 1
 SE_FLAG_FORCE_U16 = 2
 SE_FLAG_TIME_KERNELS = 4
 SE_FLAG_ALLOW_TMA = 8
+SE_FLAG_PIVOT32 = 16
+SE_FLAG_PIVOT64 = 32
 
 # Every symbol include/scaling_elections_amd.h declares.
 EXPORTED = (

@@ -7,9 +7,9 @@
 // blocked Floyd-Warshall over pivot blocks K of depth T, three phases per block, like the
 // reference's tiled kernels (scaling_elections.py:162-271, scaling_elections.cu:224-343),
 // but laid out for CDNA:
-//   * pivot   -- one workgroup closes the T x T tile (K,K) in LDS;
-//   * panels  -- rows K and columns K are each one max-min product with the closed pivot
-//                tile (no sequential dependence once the pivot is closed);
+//   * pivot + panels -- one fused kernel: every workgroup closes the T x T tile (K,K) in
+//                registers, then its chunk of rows K / columns K is one max-min product
+//                with the closed pivot tile (no sequential dependence once it is closed);
 //   * independent -- every cell outside rows/columns K gets max(c, A (x) B) with A the
 //                column panel and B the row panel: a depth-T max-min "GEMM" on 128 x 128
 //                macro tiles, 8 x 8 register micro tiles per lane, strips staged in LDS.
@@ -107,82 +107,155 @@ __global__ __launch_bounds__(256) void p0_write_kernel(const uint32_t* __restric
 }
 
 // ---------------------------------------------------------------------------------------
-// Pivot tile: closes W[K][K] (T x T) in LDS.  Within step k the cells read (row k and
-// column k) cannot change -- min(p[i][k], p[k][k]) <= p[i][k] -- so one barrier per step
-// is enough.
-
-template <typename E, uint32_t T>
-__global__ __launch_bounds__(256) void pivot_kernel(E* __restrict__ W, size_t ldw, uint32_t kb) {
-    __shared__ uint32_t P[T][T + 1];
-    for (uint32_t q = threadIdx.x; q < T * T; q += kThreads) {
-        uint32_t i = q / T, j = q % T;
-        P[i][j] = W[(size_t)(kb + i) * ldw + kb + j];
-    }
-    for (uint32_t k = 0; k < T; ++k) {
-        __syncthreads();
-        for (uint32_t q = threadIdx.x; q < T * T; q += kThreads) {
-            uint32_t i = q / T, j = q % T;
-            uint32_t m = min(P[i][k], P[k][j]);
-            if (m > P[i][j]) P[i][j] = m;
-        }
-    }
-    __syncthreads();
-    for (uint32_t q = threadIdx.x; q < T * T; q += kThreads) {
-        uint32_t i = q / T, j = q % T;
-        W[(size_t)(kb + i) * ldw + kb + j] = (E)P[i][j];
-    }
-}
-
-// ---------------------------------------------------------------------------------------
-// Panels.  With the pivot tile P closed, the row panel (K, J) becomes
-//     new[i][j] = max(old[i][j], max_k min(P[i][k], old[k][j]))
-// in one shot (split any path at its last vertex in K), and the column panel (I, K)
+// Pivot + panels, fused.  Every workgroup closes the pivot tile P = W[K][K] (T x T) itself,
+// in registers: lane t owns an R x R block (R = T / 16) at rows R (t / 16), columns
+// R (t % 16).  At step k the owners of row k and column k publish them through LDS
+// (double-buffered by k parity, so one barrier per step suffices) and every lane updates
+// its block.  The cells read in step k cannot change in step k (min(p[i][k], p[k][k]) <=
+// p[i][k]), so the step is race-free.
+// With P closed, the row panel (K, J) becomes, in one shot (split any path at its last
+// vertex in K),
+//     new[i][j] = max(old[i][j], max_k min(P[i][k], old[k][j])),
+// and the column panel (I, K)
 //     new[i][j] = max(old[i][j], max_k min(old[i][k], P[k][j])).
-// Blocks [0, nb) take 128-column chunks of the row panel, blocks [nb, 2 nb) 128-row
-// chunks of the column panel.  Cells of (K, K) are left alone (already closed).
+// Blocks [0, nb) take 128-column chunks of the row panel, [nb, 2 nb) 128-row chunks of
+// the column panel; cells of (K, K) are not written here.  Block 0 stores the closed P
+// into pivot_out; the next round's block 0 (or commit_pivot_kernel after the last round)
+// copies it into W -- nothing reads W[K][K] before then: the independent kernel of this
+// round only uses it for pivot-row/column cells, which it does not store.
 
 template <typename E, uint32_t T>
-__global__ __launch_bounds__(256) void panel_kernel(E* __restrict__ W, size_t ldw, uint32_t kb,
-                                                    uint32_t nb) {
+__global__ __launch_bounds__(256) void pivot_panel_kernel(E* __restrict__ W, size_t ldw, uint32_t kb,
+                                                          uint32_t nb, uint32_t* __restrict__ pivot_out,
+                                                          const uint32_t* __restrict__ pivot_prev,
+                                                          uint32_t kb_prev) {
+    constexpr uint32_t R = T / 16;  // pivot rows/columns per lane
+    constexpr uint32_t RP = T / 16; // panel: row-panel rows per lane / column-panel columns per lane
     __shared__ uint32_t P[T][T + 1];
-    __shared__ uint32_t S[kMacro * (T + 1)];  // the chunk, either T x 128 or 128 x T
+    __shared__ uint32_t S[kMacro * (T + 1)];  // the chunk: T x 128 (row) or 128 x (T + 1) (column)
+    __shared__ uint32_t rowbuf[2][T], colbuf[2][T];
+    const uint32_t t = threadIdx.x, ty = t >> 4, tx = t & 15;
     const bool row_panel = blockIdx.x < nb;
     const uint32_t c0 = (row_panel ? blockIdx.x : blockIdx.x - nb) * kMacro;
-    for (uint32_t q = threadIdx.x; q < T * T; q += kThreads) {
-        uint32_t i = q / T, j = q % T;
-        P[i][j] = W[(size_t)(kb + i) * ldw + kb + j];
+
+    // hand the previous round's closed pivot to W
+    if (blockIdx.x == 0 && pivot_prev != nullptr) {
+        for (uint32_t q = t; q < T * T; q += kThreads)
+            W[(size_t)(kb_prev + q / T) * ldw + kb_prev + q % T] = (E)pivot_prev[q];
     }
+
+    // stage the panel chunk (old values) while the pivot closes
     if (row_panel) {
-        for (uint32_t q = threadIdx.x; q < T * kMacro; q += kThreads) {
+        for (uint32_t q = t; q < T * kMacro; q += kThreads) {
             uint32_t i = q / kMacro, j = q % kMacro;
             S[i * kMacro + j] = W[(size_t)(kb + i) * ldw + c0 + j];
         }
     } else {
-        for (uint32_t q = threadIdx.x; q < kMacro * T; q += kThreads) {
+        for (uint32_t q = t; q < kMacro * T; q += kThreads) {
             uint32_t i = q / T, j = q % T;
             S[i * (T + 1) + j] = W[(size_t)(c0 + i) * ldw + kb + j];
         }
     }
+
+    // pivot closure in registers
+    const uint32_t pr = R * ty, pc = R * tx;
+    uint32_t cell[R][R];
+#pragma unroll
+    for (uint32_t r = 0; r < R; ++r)
+#pragma unroll
+        for (uint32_t c = 0; c < R; ++c) cell[r][c] = W[(size_t)(kb + pr + r) * ldw + kb + pc + c];
+#pragma unroll
+    for (uint32_t k = 0; k < T; ++k) {
+        const uint32_t kr = k % R, buf = k & 1;
+        if (ty == k / R) {
+#pragma unroll
+            for (uint32_t c = 0; c < R; ++c) rowbuf[buf][pc + c] = cell[kr][c];
+        }
+        if (tx == k / R) {
+#pragma unroll
+            for (uint32_t r = 0; r < R; ++r) colbuf[buf][pr + r] = cell[r][kr];
+        }
+        __syncthreads();
+        uint32_t a[R], b[R];
+#pragma unroll
+        for (uint32_t r = 0; r < R; ++r) a[r] = colbuf[buf][pr + r];
+#pragma unroll
+        for (uint32_t c = 0; c < R; ++c) b[c] = rowbuf[buf][pc + c];
+#pragma unroll
+        for (uint32_t r = 0; r < R; ++r)
+#pragma unroll
+            for (uint32_t c = 0; c < R; ++c) cell[r][c] = max(cell[r][c], min(a[r], b[c]));
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < R; ++r)
+#pragma unroll
+        for (uint32_t c = 0; c < R; ++c) {
+            P[pr + r][pc + c] = cell[r][c];
+            if (blockIdx.x == 0) pivot_out[(pr + r) * T + pc + c] = cell[r][c];
+        }
     __syncthreads();
+
     if (row_panel) {
-        for (uint32_t q = threadIdx.x; q < T * kMacro; q += kThreads) {
-            uint32_t i = q / kMacro, j = q % kMacro;
-            if (c0 + j - kb < T) continue;  // inside (K, K)
-            uint32_t acc = S[i * kMacro + j];
-#pragma unroll 8
-            for (uint32_t k = 0; k < T; ++k) acc = max(acc, min(P[i][k], S[k * kMacro + j]));
-            W[(size_t)(kb + i) * ldw + c0 + j] = (E)acc;
+        // lane: rows RP ty .. +RP-1 of the chunk, columns 8 tx .. 8 tx + 7
+        uint32_t acc[RP][8];
+#pragma unroll
+        for (uint32_t r = 0; r < RP; ++r)
+#pragma unroll
+            for (uint32_t c = 0; c < 8; ++c) acc[r][c] = S[(RP * ty + r) * kMacro + 8 * tx + c];
+#pragma unroll 4
+        for (uint32_t k = 0; k < T; ++k) {
+            uint32_t a[RP], b[8];
+#pragma unroll
+            for (uint32_t r = 0; r < RP; ++r) a[r] = P[RP * ty + r][k];
+#pragma unroll
+            for (uint32_t c = 0; c < 8; ++c) b[c] = S[k * kMacro + 8 * tx + c];
+#pragma unroll
+            for (uint32_t r = 0; r < RP; ++r)
+#pragma unroll
+                for (uint32_t c = 0; c < 8; ++c) acc[r][c] = max(acc[r][c], min(a[r], b[c]));
+        }
+        if (c0 + 8 * tx - kb >= T) {  // columns of the pivot block stay untouched
+#pragma unroll
+            for (uint32_t r = 0; r < RP; ++r)
+#pragma unroll
+                for (uint32_t c = 0; c < 8; ++c)
+                    W[(size_t)(kb + RP * ty + r) * ldw + c0 + 8 * tx + c] = (E)acc[r][c];
         }
     } else {
-        for (uint32_t q = threadIdx.x; q < kMacro * T; q += kThreads) {
-            uint32_t i = q / T, j = q % T;
-            if (c0 + i - kb < T) continue;  // inside (K, K)
-            uint32_t acc = S[i * (T + 1) + j];
-#pragma unroll 8
-            for (uint32_t k = 0; k < T; ++k) acc = max(acc, min(S[i * (T + 1) + k], P[k][j]));
-            W[(size_t)(c0 + i) * ldw + kb + j] = (E)acc;
+        // lane: rows 8 ty .. 8 ty + 7 of the chunk, columns RP tx .. +RP-1
+        uint32_t acc[8][RP];
+#pragma unroll
+        for (uint32_t r = 0; r < 8; ++r)
+#pragma unroll
+            for (uint32_t c = 0; c < RP; ++c) acc[r][c] = S[(8 * ty + r) * (T + 1) + RP * tx + c];
+#pragma unroll 4
+        for (uint32_t k = 0; k < T; ++k) {
+            uint32_t a[8], b[RP];
+#pragma unroll
+            for (uint32_t r = 0; r < 8; ++r) a[r] = S[(8 * ty + r) * (T + 1) + k];
+#pragma unroll
+            for (uint32_t c = 0; c < RP; ++c) b[c] = P[k][RP * tx + c];
+#pragma unroll
+            for (uint32_t r = 0; r < 8; ++r)
+#pragma unroll
+                for (uint32_t c = 0; c < RP; ++c) acc[r][c] = max(acc[r][c], min(a[r], b[c]));
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < 8; ++r) {
+            const uint32_t i = c0 + 8 * ty + r;
+            if (i - kb < T) continue;  // rows of the pivot block stay untouched
+#pragma unroll
+            for (uint32_t c = 0; c < RP; ++c) W[(size_t)i * ldw + kb + RP * tx + c] = (E)acc[r][c];
         }
     }
+}
+
+// Copies the last round's closed pivot tile into W.
+template <typename E, uint32_t T>
+__global__ __launch_bounds__(256) void commit_pivot_kernel(E* __restrict__ W, size_t ldw, uint32_t kb,
+                                                           const uint32_t* __restrict__ pivot) {
+    for (uint32_t q = threadIdx.x; q < T * T; q += kThreads)
+        W[(size_t)(kb + q / T) * ldw + kb + q % T] = (E)pivot[q];
 }
 
 // ---------------------------------------------------------------------------------------
@@ -396,7 +469,7 @@ struct EventTimer {
 enum { kKindInit = 0, kKindDiag = 1, kKindPanel = 2, kKindIndep = 3, kKindOut = 4 };
 
 template <typename E, uint32_t T>
-int run_rounds(E* W, size_t ldw, uint32_t n, uint32_t N, hipStream_t s, EventTimer& tm,
+int run_rounds(E* W, size_t ldw, uint32_t n, uint32_t N, uint32_t* pivot_buf, hipStream_t s, EventTimer& tm,
                uint32_t* rounds_out) {
     const uint32_t nb = N / kMacro;
     const uint32_t rounds = (n + T - 1) / T;
@@ -406,11 +479,10 @@ int run_rounds(E* W, size_t ldw, uint32_t n, uint32_t N, hipStream_t s, EventTim
             return SE_ERR_INTERRUPTED;
         }
         const uint32_t kb = r * T;
-        tm.mark(kKindDiag);
-        pivot_kernel<E, T><<<1, kThreads, 0, s>>>(W, ldw, kb);
-        tm.mark(kKindDiag);
+        uint32_t* out = pivot_buf + (r & 1) * T * T;
+        const uint32_t* prev = r ? pivot_buf + ((r - 1) & 1) * T * T : nullptr;
         tm.mark(kKindPanel);
-        panel_kernel<E, T><<<2 * nb, kThreads, 0, s>>>(W, ldw, kb, nb);
+        pivot_panel_kernel<E, T><<<2 * nb, kThreads, 0, s>>>(W, ldw, kb, nb, out, prev, kb - T);
         tm.mark(kKindPanel);
         tm.mark(kKindIndep);
         if constexpr (sizeof(E) == 2)
@@ -418,6 +490,13 @@ int run_rounds(E* W, size_t ldw, uint32_t n, uint32_t N, hipStream_t s, EventTim
         else
             indep_u32_kernel<T><<<dim3(nb, nb), kThreads, 0, s>>>(W, ldw, kb, nb);
         tm.mark(kKindIndep);
+        SE_CHECK_LAUNCH("closure round launch failed");
+    }
+    if (rounds) {
+        tm.mark(kKindPanel);
+        commit_pivot_kernel<E, T><<<1, kThreads, 0, s>>>(W, ldw, (rounds - 1) * T,
+                                                          pivot_buf + ((rounds - 1) & 1) * T * T);
+        tm.mark(kKindPanel);
         SE_CHECK_LAUNCH("closure round launch failed");
     }
     *rounds_out = rounds;
@@ -484,10 +563,21 @@ int closure_device_impl(DeviceContext* ctx, const uint32_t* d, uint32_t n, size_
     tm.mark(kKindInit);
     SE_CHECK_LAUNCH("p0 init launch failed");
 
-    constexpr uint32_t T = 32;
+    // pivot depth: deeper pivots halve the HBM traffic per cell update of the independent
+    // kernel (2 s / T bytes per update); 64 fits the 16-bit strips in 34 KB of LDS
+    uint32_t T = use16 ? 64 : 32;
+    if (flags & SE_FLAG_PIVOT32) T = 32;
+    if (flags & SE_FLAG_PIVOT64) T = 64;
+    uint32_t* pivot_buf = static_cast<uint32_t*>(scratch(ctx, kSlotPivot, 2 * 64 * 64 * sizeof(uint32_t)));
+    if (!pivot_buf) return SE_ERR_ALLOC;
     uint32_t rounds = 0;
-    int rc = use16 ? run_rounds<uint16_t, T>(static_cast<uint16_t*>(W), ldw, n, N, s, tm, &rounds)
-                   : run_rounds<uint32_t, T>(static_cast<uint32_t*>(W), ldw, n, N, s, tm, &rounds);
+    int rc;
+    if (use16)
+        rc = T == 64 ? run_rounds<uint16_t, 64>(static_cast<uint16_t*>(W), ldw, n, N, pivot_buf, s, tm, &rounds)
+                     : run_rounds<uint16_t, 32>(static_cast<uint16_t*>(W), ldw, n, N, pivot_buf, s, tm, &rounds);
+    else
+        rc = T == 64 ? run_rounds<uint32_t, 64>(static_cast<uint32_t*>(W), ldw, n, N, pivot_buf, s, tm, &rounds)
+                     : run_rounds<uint32_t, 32>(static_cast<uint32_t*>(W), ldw, n, N, pivot_buf, s, tm, &rounds);
     if (rc != SE_OK) return rc;
 
     dim3 go(n);

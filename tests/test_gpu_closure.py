@@ -79,6 +79,22 @@ def test_width_selection_is_exact(gpu_lib, oracle_mod, hi, width):
             _timed(d, _lib.SE_FLAG_FORCE_U16)
 
 
+@pytest.mark.parametrize("width", ["u16", "u32"])
+@pytest.mark.parametrize("pivot", [32, 64])
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 200, 511, 640])
+def test_every_width_and_pivot_depth(gpu_lib, oracle_mod, width, pivot, n):
+    from scalingelections_amd import _lib
+
+    rng = np.random.default_rng(n * 7 + pivot)
+    d = rng.integers(0, 60000, (n, n)).astype(np.uint32)
+    flags = (_lib.SE_FLAG_FORCE_U16 if width == "u16" else _lib.SE_FLAG_FORCE_U32)
+    flags |= _lib.SE_FLAG_PIVOT32 if pivot == 32 else _lib.SE_FLAG_PIVOT64
+    got, t = _timed(d, flags)
+    assert (t["pivot"], t["width_bits"]) == (pivot, 16 if width == "u16" else 32)
+    assert t["rounds"] == (n + pivot - 1) // pivot
+    assert np.array_equal(got, oracle_mod.closure_tiled(d, 16))
+
+
 def test_degenerate_inputs(gpu_lib, oracle_mod):
     for d in (np.zeros((64, 64), np.uint32), np.full((70, 70), 7, np.uint32),
               np.triu(np.full((130, 130), 9, np.uint32), 1), np.eye(40, dtype=np.uint32) * 5):
@@ -111,7 +127,7 @@ def test_device_entry_point_with_caller_buffers(gpu_lib, oracle_mod):
     s.synchronize()
     host = out.download(np.empty((n, n + 3), dtype=np.uint32))
     assert np.array_equal(host[:, :n], want)
-    assert t.total_ms > 0 and t.indep_ms > 0 and t.rounds == (n + 31) // 32
+    assert t.total_ms > 0 and t.indep_ms > 0 and t.rounds == (n + t.pivot - 1) // t.pivot
     # in place: output over the input buffer
     rc = _lib.lib().se_closure_device(dev.ptr, n, n, dev.ptr, n, 0, None, None)
     _lib.check(rc)
