@@ -124,6 +124,43 @@ int se_rank(const uint32_t *paths, uint32_t n, size_t row_stride, uint32_t *wins
 int se_rank_device(const uint32_t *paths_dev, uint32_t n, size_t row_stride, uint32_t *wins_host,
                    uint32_t *ranking_host, void *stream);
 
+/* ---- batches of independent elections -------------------------------------------------- */
+/* No single reference counterpart: the reference handles one election per call
+ * (build_pairwise_preferences + compute_strongest_paths + get_winner_and_ranking,
+ * scaling_elections.py:51-296); this entry point runs the same three steps for many
+ * elections at once and returns, per election, exactly what those calls return.
+ * candidates/voters: host arrays of `elections` entries (C_e <= 1024; m_e voters).
+ * positions_dev: concatenation over e of m_e x C_e int32 position arrays (strict orders).
+ * paths_dev: concatenation of C_e x C_e uint32 strongest-path matrices (zero diagonal).
+ * ranking_dev: concatenation of C_e candidate indices by descending wins, ties ascending.
+ * Default strategy: two launches for the whole batch (counting work items, then one
+ * workgroup per election for p0, closure, wins and ranking).  SE_BATCH_PER_LAUNCH runs
+ * every election through the single-election path instead (for comparison). */
+#define SE_BATCH_PER_LAUNCH 64u
+
+typedef struct se_batch_timing {
+    double total_ms;          /* counting + closure                                    */
+    double count_ms;          /* counting launch (batched) / all per-election work     */
+    double closure_ms;        /* closure launch (batched)                              */
+    uint32_t elections;
+    uint32_t reserved;
+    uint64_t cells;           /* sum of C^3                                            */
+    uint64_t pair_votes;      /* sum of m * C * (C - 1) / 2                            */
+} se_batch_timing;
+
+int se_batch_device(uint32_t elections, const uint32_t *candidates, const uint32_t *voters,
+                    const int32_t *positions_dev, uint32_t *paths_dev, uint32_t *ranking_dev,
+                    uint32_t flags, void *stream, se_batch_timing *timing);
+
+/* ---- workload generation (no reference counterpart) ------------------------------------ */
+/* Mallows ballots by repeated insertion (item k of the reference order inserted at
+ * position j <= k with probability proportional to phi^(k - j)), seeded per (seed,
+ * election, voter); writes positions in the se_batch_device layout.  reference: the
+ * concatenated reference orders (C_e entries each).  threads <= 0: up to 16. */
+int se_synth_mallows(uint32_t elections, const uint32_t *candidates, const uint32_t *voters,
+                     const double *phi, const int32_t *reference, uint64_t seed, int32_t *positions,
+                     int threads);
+
 /* ---- misc ---------------------------------------------------------------------------- */
 /* Sequential float sum, the reference's non-CUDA reduce() (std::accumulate order). */
 float se_reduce_f32(const float *data, size_t count);

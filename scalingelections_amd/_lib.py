@@ -38,7 +38,25 @@ EXPORTED = (
     "se_count", "se_count_device", "se_count_rankings",
     "se_rank", "se_rank_device", "se_reduce_f32",
     "se_interrupt", "se_clear_interrupt", "se_install_sigint", "se_restore_sigint",
+    "se_batch_device", "se_synth_mallows",
 )
+
+SE_BATCH_PER_LAUNCH = 64
+
+
+class se_batch_timing(ctypes.Structure):
+    _fields_ = [
+        ("total_ms", ctypes.c_double),
+        ("count_ms", ctypes.c_double),
+        ("closure_ms", ctypes.c_double),
+        ("elections", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("cells", ctypes.c_uint64),
+        ("pair_votes", ctypes.c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
 
 
 class se_timing(ctypes.Structure):
@@ -112,7 +130,10 @@ def _declare(L):
     L.se_install_sigint.restype = c.c_int
     L.se_restore_sigint.argtypes = []
     L.se_restore_sigint.restype = None
-    # reserved for entry points added below this list
+    L.se_batch_device.argtypes = [u32, vp, vp, vp, vp, vp, u32, vp, c.POINTER(se_batch_timing)]
+    L.se_batch_device.restype = c.c_int
+    L.se_synth_mallows.argtypes = [u32, vp, vp, vp, vp, c.c_uint64, vp, c.c_int]
+    L.se_synth_mallows.restype = c.c_int
     del i32, i64
 
 

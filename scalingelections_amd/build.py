@@ -27,7 +27,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
-HOST_ONLY = {"cpu_closure.cpp": ["-march=x86-64-v3"]}
+HOST_ONLY = {"cpu_closure.cpp": ["-march=x86-64-v3"], "synth.cpp": ["-march=x86-64-v3"]}
 
 
 def _hipcc() -> str:

@@ -24,6 +24,8 @@ int count_rankings_device_impl(DeviceContext* ctx, const int32_t* ids, const int
                                uint32_t m, uint32_t n, uint32_t* d, size_t ldd, hipStream_t s);
 int wins_device_impl(DeviceContext* ctx, const uint32_t* p, uint32_t n, size_t ld, uint32_t* wins,
                      hipStream_t s);
+int batch_device_impl(DeviceContext* ctx, uint32_t E, const uint32_t* Cs, const uint32_t* ms, const int32_t* pos,
+                      uint32_t* paths, uint32_t* ranking, uint32_t flags, hipStream_t s, se_batch_timing* timing);
 int closure_cpu_impl(const uint32_t* d, uint32_t n, size_t ld, uint32_t* p, uint32_t tile, int threads);
 int closure_cpu_serial_impl(const uint32_t* d, uint32_t n, size_t ld, uint32_t* p);
 
@@ -389,6 +391,29 @@ int se_rank(const uint32_t* paths, uint32_t n, size_t row_stride, uint32_t* wins
         return SE_ERR_COPY;
     if (ranking) stable_rank(wins, n, ranking);
     return SE_OK;
+}
+
+int se_batch_device(uint32_t elections, const uint32_t* candidates, const uint32_t* voters,
+                    const int32_t* positions_dev, uint32_t* paths_dev, uint32_t* ranking_dev, uint32_t flags,
+                    void* stream, se_batch_timing* timing) {
+    clear_error();
+    if (timing) *timing = se_batch_timing{};
+    if (elections == 0) return SE_OK;
+    if (!candidates || !voters || !paths_dev || !ranking_dev) {
+        set_error("invalid batch buffers");
+        return SE_ERR_INVALID;
+    }
+    uint64_t total_pos = 0;
+    for (uint32_t e = 0; e < elections; ++e) total_pos += (uint64_t)candidates[e] * voters[e];
+    if (total_pos && !positions_dev) {
+        set_error("invalid batch buffers");
+        return SE_ERR_INVALID;
+    }
+    DeviceContext* ctx = current_context();
+    if (!ctx) return SE_ERR_NO_DEVICE;
+    std::lock_guard<std::mutex> g(ctx->lock);
+    return batch_device_impl(ctx, elections, candidates, voters, positions_dev, paths_dev, ranking_dev, flags,
+                             pick_stream(ctx, stream), timing);
 }
 
 float se_reduce_f32(const float* data, size_t count) {

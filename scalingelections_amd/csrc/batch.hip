@@ -1,0 +1,371 @@
+// batch.hip -- thousands of independent elections of uneven size in two launches.
+//
+// Per election e (C_e candidates, m_e voters, positions m_e x C_e int32) the batch
+// produces exactly what the single-election path produces: the strongest-path matrix
+// (scaling_elections.py:91-125 semantics, zero diagonal) and the ranking by descending
+// wins with ties in ascending index (:274-296).
+//
+// Launch 1, batch_count_kernel: one workgroup per work item, items sorted by decreasing
+//   cost so the longest start first.  Elections with C <= 128 are one item each, counted
+//   voter-parallel: the C x C counter matrix is cut into 8 x 8 blocks (<= 256) and the
+//   256 lanes are split into G = 256 / blocks voter groups; partial counts are summed
+//   through LDS.  Larger elections are cut into 128 x 128 tile pairs (I <= J), each an
+//   item, counted like the single-election kernel (count_tile), d[j][i] = m - d[i][j].
+// Launch 2, batch_closure_kernel: one workgroup per election, largest first.  p0 into a
+//   zero-padded work matrix (global memory, L2-resident at these sizes), then the blocked
+//   closure run by the workgroup alone (close_pivot, panel chunk products, independent
+//   macro tiles -- the same building blocks as the single-election kernels, with barriers
+//   in place of kernel boundaries), then output, win counts and the stable ranking, all
+//   on the device.  Elections with C <= 64 are closed entirely in registers/LDS.
+//
+// The per-launch strategy (SE_BATCH_PER_LAUNCH) runs every election through the plain
+// single-election entry points instead, for comparison.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <numeric>
+#include <vector>
+
+#include "closure_tiles.h"
+#include "count_tiles.h"
+#include "se_internal.h"
+
+namespace se {
+
+int count_device_impl(DeviceContext* ctx, const int32_t* pos, uint32_t m, uint32_t n, uint32_t* d, size_t ldd,
+                      hipStream_t s, double* kernel_ms);
+int closure_device_impl(DeviceContext* ctx, const uint32_t* d, uint32_t n, size_t ld, uint32_t* out,
+                        size_t ldo, uint32_t flags, hipStream_t s, se_timing* timing);
+int wins_device_impl(DeviceContext* ctx, const uint32_t* p, uint32_t n, size_t ld, uint32_t* wins,
+                     hipStream_t s);
+
+namespace {
+
+constexpr uint32_t kMaxBatchC = 1024;  // larger elections belong on the single-election path
+
+struct ElectionMeta {
+    uint64_t pos_off;  // element offset of the positions (m x C int32)
+    uint64_t d_off;    // element offset of d and of p (C x C uint32)
+    uint64_t w_off;    // element offset of the work matrix (Cp x Cp)
+    uint64_t r_off;    // element offset of the ranking (C uint32)
+    uint32_t C, m, Cp, pad;
+};
+
+// ------------------------------------------------------------------------- counting
+// item.x = election, item.y = I, item.z = J, item.w = mode (0: small voter-parallel, 1: tile)
+__global__ __launch_bounds__(256) void batch_count_kernel(const int32_t* __restrict__ pos_all,
+                                                          uint32_t* __restrict__ d_all,
+                                                          const ElectionMeta* __restrict__ meta,
+                                                          const uint4* __restrict__ items) {
+    __shared__ __align__(16) int32_t Pi[kVC][kCT];
+    __shared__ __align__(16) int32_t Pj[kVC][kCT];
+    const uint4 it = items[blockIdx.x];
+    const ElectionMeta em = meta[it.x];
+    const int32_t* pos = pos_all + em.pos_off;
+    uint32_t* d = d_all + em.d_off;
+    const uint32_t C = em.C, m = em.m;
+    if (it.w == 1) {
+        count_tile(pos, m, C, C, d, C, it.y, it.z, Pi, Pj);
+        return;
+    }
+    // voter-parallel small election
+    const uint32_t t = threadIdx.x;
+    const uint32_t nblk = (C + 7) / 8, blocks = nblk * nblk, G = kCThreads / blocks;
+    const uint32_t b = t % blocks, g = t / blocks;
+    const uint32_t bi = b / nblk, bj = b % nblk;
+    uint32_t acc[8][8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[a][c] = 0;
+    if (g < G) {
+        for (uint32_t v = g; v < m; v += G) {
+            const int32_t* row = pos + (size_t)v * C;
+            int32_t pi[8], pj[8];
+#pragma unroll
+            for (int a = 0; a < 8; ++a) {
+                const uint32_t ci = 8 * bi + a, cj = 8 * bj + a;
+                pi[a] = ci < C ? row[ci] : 0;
+                pj[a] = cj < C ? row[cj] : 0;
+            }
+#pragma unroll
+            for (int a = 0; a < 8; ++a)
+#pragma unroll
+                for (int c = 0; c < 8; ++c) acc[a][c] += (uint32_t)(pi[a] < pj[c]);
+        }
+    }
+    // sum the G partial blocks, 8 counters at a time, through LDS (Pi reused: 256 x 8 words)
+    int32_t* red = &Pi[0][0];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < 8; ++c) red[t * 8 + c] = (int32_t)acc[a][c];
+        __syncthreads();
+        for (uint32_t q = t; q < blocks * 8; q += kCThreads) {
+            const uint32_t bb = q / 8, c = q % 8;
+            uint32_t sum = 0;
+            for (uint32_t gg = 0; gg < G; ++gg) sum += (uint32_t)red[(gg * blocks + bb) * 8 + c];
+            const uint32_t i = 8 * (bb / nblk) + a, j = 8 * (bb % nblk) + c;
+            if (i < C && j < C) d[(size_t)i * C + j] = sum;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------- closure
+template <typename E, uint32_t T>
+struct BatchLds {
+    uint32_t P[T][T + 1];
+    uint32_t rowbuf[2][T], colbuf[2][T];
+    uint32_t wins[kMaxBatchC];
+    union alignas(16) {
+        uint32_t S[kMacro * (T + 1)];
+        E indep[sizeof(E) == 2 ? indep_u16_lds_elems<T>() : indep_u32_lds_elems<T>()];
+    } u;
+};
+
+template <typename E, uint32_t T>
+__device__ __forceinline__ void indep_tile(E* W, size_t ldw, uint32_t kb, uint32_t I0, uint32_t J0, E* lds) {
+    if constexpr (sizeof(E) == 2)
+        indep_tile_u16<T>(W, ldw, kb, I0, J0, lds);
+    else
+        indep_tile_u32<T>(W, ldw, kb, I0, J0, lds);
+}
+
+template <typename E, uint32_t T>
+__global__ __launch_bounds__(256) void batch_closure_kernel(const uint32_t* __restrict__ d_all,
+                                                            E* __restrict__ W_all, uint32_t* __restrict__ p_all,
+                                                            uint32_t* __restrict__ rank_all,
+                                                            const ElectionMeta* __restrict__ meta,
+                                                            const uint32_t* __restrict__ order) {
+    __shared__ __align__(16) BatchLds<E, T> L;
+    const uint32_t t = threadIdx.x;
+    const ElectionMeta em = meta[order[blockIdx.x]];
+    const uint32_t C = em.C, Cp = em.Cp;
+    const uint32_t* d = d_all + em.d_off;
+    uint32_t* p = p_all + em.d_off;
+    E* W = W_all + em.w_off;
+    if (C == 0) return;
+
+    // p0 (scaling_elections.py:103-112) into the padded work matrix
+    for (uint32_t q = t; q < Cp * Cp; q += kThreads) {
+        const uint32_t i = q / Cp, j = q % Cp;
+        uint32_t v = 0;
+        if (i < C && j < C && i != j) {
+            const uint32_t x = d[(size_t)i * C + j];
+            if (x > d[(size_t)j * C + i]) v = x;
+        }
+        W[q] = (E)v;
+    }
+    __syncthreads();
+
+    const uint32_t rounds = (C + T - 1) / T;
+    for (uint32_t r = 0; r < rounds; ++r) {
+        const uint32_t kb = r * T;
+        close_pivot<E, T>(W, Cp, kb, min(T, C - kb), L.P, L.rowbuf, L.colbuf);
+        if (Cp > T) {
+            for (uint32_t c0 = 0; c0 < Cp; c0 += kMacro) {
+                load_row_chunk<E, T>(W, Cp, kb, c0, L.u.S);
+                __syncthreads();
+                row_chunk_product<E, T>(W, Cp, kb, c0, L.P, L.u.S);
+                __syncthreads();
+                load_col_chunk<E, T>(W, Cp, kb, c0, L.u.S);
+                __syncthreads();
+                col_chunk_product<E, T>(W, Cp, kb, c0, L.P, L.u.S);
+                __syncthreads();
+            }
+        }
+        for (uint32_t q = t; q < T * T; q += kThreads) W[(size_t)(kb + q / T) * Cp + kb + q % T] = (E)L.P[q / T][q % T];
+        __syncthreads();
+        if (Cp > T) {
+            for (uint32_t I0 = 0; I0 < Cp; I0 += kMacro)
+                for (uint32_t J0 = 0; J0 < Cp; J0 += kMacro) indep_tile<E, T>(W, Cp, kb, I0, J0, L.u.indep);
+            __syncthreads();
+        }
+    }
+
+    // output with zero diagonal, then wins (:287-290) and the stable ranking (:292)
+    for (uint32_t q = t; q < C * C; q += kThreads) {
+        const uint32_t i = q / C, j = q % C;
+        p[q] = i == j ? 0u : (uint32_t)W[(size_t)i * Cp + j];
+    }
+    for (uint32_t i = t; i < C; i += kThreads) {
+        uint32_t w = 0;
+        for (uint32_t j = 0; j < C; ++j)
+            if (j != i && W[(size_t)i * Cp + j] > W[(size_t)j * Cp + i]) ++w;
+        L.wins[i] = w;
+    }
+    __syncthreads();
+    for (uint32_t i = t; i < C; i += kThreads) {
+        const uint32_t wi = L.wins[i];
+        uint32_t rank = 0;
+        for (uint32_t j = 0; j < C; ++j) {
+            const uint32_t wj = L.wins[j];
+            rank += (wj > wi || (wj == wi && j < i)) ? 1u : 0u;
+        }
+        rank_all[em.r_off + rank] = i;
+    }
+}
+
+inline uint32_t padded_side(uint32_t C, uint32_t T) { return C <= T ? T : (uint32_t)round_up(C, kMacro); }
+
+}  // namespace
+
+int batch_device_impl(DeviceContext* ctx, uint32_t E, const uint32_t* Cs, const uint32_t* ms, const int32_t* pos,
+                      uint32_t* paths, uint32_t* ranking, uint32_t flags, hipStream_t s, se_batch_timing* timing) {
+    uint64_t pos_total = 0, d_total = 0, r_total = 0;
+    uint32_t max_m = 0;
+    for (uint32_t e = 0; e < E; ++e) {
+        if (Cs[e] > kMaxBatchC) {
+            set_error("batched elections are limited to 1024 candidates; use se_closure_device for larger ones");
+            return SE_ERR_INVALID;
+        }
+        pos_total += (uint64_t)ms[e] * Cs[e];
+        d_total += (uint64_t)Cs[e] * Cs[e];
+        r_total += Cs[e];
+        max_m = std::max(max_m, ms[e]);
+    }
+    hipEvent_t ev[3] = {};
+    if (timing) {
+        for (auto& x : ev)
+            if (!hip_ok(hipEventCreate(&x), "Failed to create events")) return SE_ERR_DEVICE;
+        (void)hipEventRecord(ev[0], s);
+    }
+
+    if (flags & SE_BATCH_PER_LAUNCH) {
+        // the plain form: one count launch, one closure pipeline and one ranking per election
+        uint32_t maxC = 0;
+        for (uint32_t e = 0; e < E; ++e) maxC = std::max(maxC, Cs[e]);
+        uint32_t* dbuf = static_cast<uint32_t*>(scratch(ctx, kSlotBatchD, (size_t)maxC * maxC * 4 + 256));
+        uint32_t* wins = static_cast<uint32_t*>(scratch(ctx, kSlotBatchMeta, (size_t)maxC * 4 + 256));
+        if (!dbuf || !wins) return SE_ERR_ALLOC;
+        std::vector<uint32_t> hw(maxC), hr(maxC);
+        uint64_t po = 0, doff = 0, ro = 0;
+        for (uint32_t e = 0; e < E; ++e) {
+            const uint32_t C = Cs[e], m = ms[e];
+            if (C) {
+                int rc = count_device_impl(ctx, pos + po, m, C, dbuf, C, s, nullptr);
+                if (rc == SE_OK) rc = closure_device_impl(ctx, dbuf, C, C, paths + doff, C, 0, s, nullptr);
+                if (rc == SE_OK) rc = wins_device_impl(ctx, paths + doff, C, C, wins, s);
+                if (rc != SE_OK) return rc;
+                if (!hip_ok(hipMemcpyAsync(hw.data(), wins, C * 4, hipMemcpyDeviceToHost, s), "Failed to copy data from device to host") ||
+                    !hip_ok(hipStreamSynchronize(s), "CUDA operations did not complete successfully"))
+                    return SE_ERR_COPY;
+                std::iota(hr.begin(), hr.begin() + C, 0u);
+                std::stable_sort(hr.begin(), hr.begin() + C, [&](uint32_t a, uint32_t b) { return hw[a] > hw[b]; });
+                if (!hip_ok(hipMemcpyAsync(ranking + ro, hr.data(), C * 4, hipMemcpyHostToDevice, s),
+                            "Failed to copy data from host to device") ||
+                    !hip_ok(hipStreamSynchronize(s), "CUDA operations did not complete successfully"))
+                    return SE_ERR_COPY;
+            }
+            po += (uint64_t)m * C;
+            doff += (uint64_t)C * C;
+            ro += C;
+        }
+        if (timing) {
+            (void)hipEventRecord(ev[1], s);
+            (void)hipEventRecord(ev[2], s);
+        }
+    } else {
+        const bool use16 = max_m <= 0xFFFFu;  // d <= m, so every p0 value fits 16 bits
+        const uint32_t T = use16 ? 64 : 32;
+        std::vector<ElectionMeta> meta(E);
+        uint64_t po = 0, doff = 0, wo = 0, ro = 0;
+        for (uint32_t e = 0; e < E; ++e) {
+            ElectionMeta& em = meta[e];
+            em.C = Cs[e];
+            em.m = ms[e];
+            em.Cp = em.C ? padded_side(em.C, T) : 0;
+            em.pad = 0;
+            em.pos_off = po;
+            em.d_off = doff;
+            em.w_off = wo;
+            em.r_off = ro;
+            po += (uint64_t)em.m * em.C;
+            doff += (uint64_t)em.C * em.C;
+            wo += (uint64_t)em.Cp * em.Cp;
+            ro += em.C;
+        }
+        // work items, most expensive first
+        std::vector<uint4> items;
+        std::vector<double> cost;
+        for (uint32_t e = 0; e < E; ++e) {
+            const uint32_t C = Cs[e];
+            if (!C) continue;
+            if (C <= kCT) {
+                items.push_back(make_uint4(e, 0, 0, 0));
+                cost.push_back((double)ms[e] * C * C / 256.0 + 64.0);
+            } else {
+                const uint32_t nb = (C + kCT - 1) / kCT;
+                for (uint32_t I = 0; I < nb; ++I)
+                    for (uint32_t J = I; J < nb; ++J) {
+                        items.push_back(make_uint4(e, I, J, 1));
+                        cost.push_back((double)ms[e] * 64.0 + 64.0);
+                    }
+            }
+        }
+        std::vector<uint32_t> idx(items.size());
+        std::iota(idx.begin(), idx.end(), 0u);
+        std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
+        std::vector<uint4> sorted_items(items.size());
+        for (size_t i = 0; i < idx.size(); ++i) sorted_items[i] = items[idx[i]];
+        std::vector<uint32_t> order(E);
+        std::iota(order.begin(), order.end(), 0u);
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return Cs[a] > Cs[b]; });
+
+        const size_t meta_bytes = round_up(E * sizeof(ElectionMeta), 256);
+        const size_t item_bytes = round_up(std::max<size_t>(1, sorted_items.size()) * sizeof(uint4), 256);
+        const size_t order_bytes = round_up((size_t)E * 4, 256);
+        char* mbuf = static_cast<char*>(scratch(ctx, kSlotBatchMeta, meta_bytes + item_bytes + order_bytes));
+        uint32_t* dbuf = static_cast<uint32_t*>(scratch(ctx, kSlotBatchD, d_total * 4 + 256));
+        void* wbuf = scratch(ctx, kSlotWork, wo * (use16 ? 2 : 4) + 256);
+        if (!mbuf || !dbuf || !wbuf) return SE_ERR_ALLOC;
+        ElectionMeta* dmeta = reinterpret_cast<ElectionMeta*>(mbuf);
+        uint4* ditems = reinterpret_cast<uint4*>(mbuf + meta_bytes);
+        uint32_t* dorder = reinterpret_cast<uint32_t*>(mbuf + meta_bytes + item_bytes);
+        if (!hip_ok(hipMemcpyAsync(dmeta, meta.data(), E * sizeof(ElectionMeta), hipMemcpyHostToDevice, s),
+                    "Failed to copy data from host to device") ||
+            (!sorted_items.empty() &&
+             !hip_ok(hipMemcpyAsync(ditems, sorted_items.data(), sorted_items.size() * sizeof(uint4),
+                                    hipMemcpyHostToDevice, s),
+                     "Failed to copy data from host to device")) ||
+            !hip_ok(hipMemcpyAsync(dorder, order.data(), (size_t)E * 4, hipMemcpyHostToDevice, s),
+                    "Failed to copy data from host to device"))
+            return SE_ERR_COPY;
+        if (!sorted_items.empty())
+            batch_count_kernel<<<(uint32_t)sorted_items.size(), kCThreads, 0, s>>>(pos, dbuf, dmeta, ditems);
+        SE_CHECK_LAUNCH("batch counter launch failed");
+        if (timing) (void)hipEventRecord(ev[1], s);
+        if (E) {
+            if (use16)
+                batch_closure_kernel<uint16_t, 64>
+                    <<<E, kThreads, 0, s>>>(dbuf, static_cast<uint16_t*>(wbuf), paths, ranking, dmeta, dorder);
+            else
+                batch_closure_kernel<uint32_t, 32>
+                    <<<E, kThreads, 0, s>>>(dbuf, static_cast<uint32_t*>(wbuf), paths, ranking, dmeta, dorder);
+        }
+        SE_CHECK_LAUNCH("batch closure launch failed");
+        if (timing) (void)hipEventRecord(ev[2], s);
+        // the host metadata vectors die here; the copies above must have left them first
+        if (!hip_ok(hipStreamSynchronize(s), "CUDA operations did not complete successfully")) return SE_ERR_DEVICE;
+    }
+    if (timing) {
+        if (!hip_ok(hipEventSynchronize(ev[2]), "CUDA operations did not complete successfully")) return SE_ERR_DEVICE;
+        float a = 0.f, b = 0.f;
+        (void)hipEventElapsedTime(&a, ev[0], ev[1]);
+        (void)hipEventElapsedTime(&b, ev[1], ev[2]);
+        timing->count_ms = a;
+        timing->closure_ms = b;
+        timing->total_ms = a + b;
+        timing->elections = E;
+        timing->cells = 0;
+        for (uint32_t e = 0; e < E; ++e) timing->cells += (uint64_t)Cs[e] * Cs[e] * Cs[e];
+        timing->pair_votes = 0;
+        for (uint32_t e = 0; e < E; ++e) timing->pair_votes += (uint64_t)ms[e] * Cs[e] * (Cs[e] ? Cs[e] - 1 : 0) / 2;
+        for (auto x : ev) (void)hipEventDestroy(x);
+    }
+    return SE_OK;
+}
+
+}  // namespace se

@@ -1,0 +1,337 @@
+// closure_tiles.h -- workgroup-level building blocks of the blocked max-min closure,
+// shared by the single-election kernels (closure.hip) and the batched kernels (batch.hip).
+// Every function is called by all 256 lanes of a workgroup with uniform arguments.
+//
+// Semantics (reference: scaling_elections.py:115-123): a cell update is
+//     c = max(c, min(a, b)),
+// evaluated without the reference's diagonal masks, which is exact: an update of p[i][j]
+// through pivot i or j is min(p[i][i], p[i][j]) or min(p[i][j], p[j][j]) and never raises
+// p[i][j]; the diagonal is zeroed when results are written out.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace se {
+
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+constexpr uint32_t kMacro = 128;  // macro tile side of the independent kernel
+constexpr uint32_t kThreads = 256;
+
+__device__ __forceinline__ u16x2 as_u16x2(uint32_t x) { return __builtin_bit_cast(u16x2, x); }
+__device__ __forceinline__ uint32_t as_u32(u16x2 x) { return __builtin_bit_cast(uint32_t, x); }
+__device__ __forceinline__ u16x2 pk_min(u16x2 a, u16x2 b) { return __builtin_elementwise_min(a, b); }
+__device__ __forceinline__ u16x2 pk_max(u16x2 a, u16x2 b) { return __builtin_elementwise_max(a, b); }
+
+__device__ __forceinline__ uint32_t word(const uint4& v, int i) {
+    return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
+}
+
+
+// ---------------------------------------------------------------------------------------
+// Pivot tile closure, in registers.  Lane t owns an R x R block (R = T / 16) at rows
+// R (t / 16), columns R (t % 16).  At step k the owners of row k and column k publish them
+// through LDS (double-buffered by k parity, so one barrier per step suffices) and every lane
+// updates its block.  The cells read in step k cannot change in step k
+// (min(p[i][k], p[k][k]) <= p[i][k]), so the step is race-free.  Only the first `steps`
+// pivots run (pivots past the last real candidate are no-ops on zero padding).  The closed
+// tile lands in P (LDS); the function ends with a barrier.
+template <typename E, uint32_t T>
+__device__ __forceinline__ void close_pivot(const E* __restrict__ W, size_t ldw, uint32_t kb, uint32_t steps,
+                                            uint32_t (*P)[T + 1], uint32_t (*rowbuf)[T],
+                                            uint32_t (*colbuf)[T]) {
+    constexpr uint32_t R = T / 16;
+    const uint32_t t = threadIdx.x, ty = t >> 4, tx = t & 15;
+    const uint32_t pr = R * ty, pc = R * tx;
+    uint32_t cell[R][R];
+#pragma unroll
+    for (uint32_t r = 0; r < R; ++r)
+#pragma unroll
+        for (uint32_t c = 0; c < R; ++c) cell[r][c] = W[(size_t)(kb + pr + r) * ldw + kb + pc + c];
+#pragma unroll
+    for (uint32_t k = 0; k < T; ++k) {
+        if (k >= steps) break;
+        const uint32_t kr = k % R, buf = k & 1;
+        if (ty == k / R) {
+#pragma unroll
+            for (uint32_t c = 0; c < R; ++c) rowbuf[buf][pc + c] = cell[kr][c];
+        }
+        if (tx == k / R) {
+#pragma unroll
+            for (uint32_t r = 0; r < R; ++r) colbuf[buf][pr + r] = cell[r][kr];
+        }
+        __syncthreads();
+        uint32_t a[R], b[R];
+#pragma unroll
+        for (uint32_t r = 0; r < R; ++r) a[r] = colbuf[buf][pr + r];
+#pragma unroll
+        for (uint32_t c = 0; c < R; ++c) b[c] = rowbuf[buf][pc + c];
+#pragma unroll
+        for (uint32_t r = 0; r < R; ++r)
+#pragma unroll
+            for (uint32_t c = 0; c < R; ++c) cell[r][c] = max(cell[r][c], min(a[r], b[c]));
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < R; ++r)
+#pragma unroll
+        for (uint32_t c = 0; c < R; ++c) P[pr + r][pc + c] = cell[r][c];
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------------------
+// Panels.  With the pivot tile P closed, the row panel (K, J) becomes, in one shot (split
+// any path at its last vertex in K),
+//     new[i][j] = max(old[i][j], max_k min(P[i][k], old[k][j])),
+// and the column panel (I, K)
+//     new[i][j] = max(old[i][j], max_k min(old[i][k], P[k][j])).
+// A chunk is 128 columns of the row panel or 128 rows of the column panel; cells inside
+// (K, K) are never written.  S holds the chunk's old values (T x 128, or 128 x (T + 1)).
+
+template <typename E, uint32_t T>
+__device__ __forceinline__ void load_row_chunk(const E* __restrict__ W, size_t ldw, uint32_t kb, uint32_t c0,
+                                               uint32_t* S) {
+    for (uint32_t q = threadIdx.x; q < T * kMacro; q += kThreads) {
+        uint32_t i = q / kMacro, j = q % kMacro;
+        S[i * kMacro + j] = W[(size_t)(kb + i) * ldw + c0 + j];
+    }
+}
+
+template <typename E, uint32_t T>
+__device__ __forceinline__ void load_col_chunk(const E* __restrict__ W, size_t ldw, uint32_t kb, uint32_t c0,
+                                               uint32_t* S) {
+    for (uint32_t q = threadIdx.x; q < kMacro * T; q += kThreads) {
+        uint32_t i = q / T, j = q % T;
+        S[i * (T + 1) + j] = W[(size_t)(c0 + i) * ldw + kb + j];
+    }
+}
+
+// lane: rows (T/16) ty .. of the chunk, columns 8 tx .. 8 tx + 7
+template <typename E, uint32_t T>
+__device__ __forceinline__ void row_chunk_product(E* __restrict__ W, size_t ldw, uint32_t kb, uint32_t c0,
+                                                  const uint32_t (*P)[T + 1], const uint32_t* S) {
+    constexpr uint32_t RP = T / 16;
+    const uint32_t t = threadIdx.x, ty = t >> 4, tx = t & 15;
+    uint32_t acc[RP][8];
+#pragma unroll
+    for (uint32_t r = 0; r < RP; ++r)
+#pragma unroll
+        for (uint32_t c = 0; c < 8; ++c) acc[r][c] = S[(RP * ty + r) * kMacro + 8 * tx + c];
+#pragma unroll 4
+    for (uint32_t k = 0; k < T; ++k) {
+        uint32_t a[RP], b[8];
+#pragma unroll
+        for (uint32_t r = 0; r < RP; ++r) a[r] = P[RP * ty + r][k];
+#pragma unroll
+        for (uint32_t c = 0; c < 8; ++c) b[c] = S[k * kMacro + 8 * tx + c];
+#pragma unroll
+        for (uint32_t r = 0; r < RP; ++r)
+#pragma unroll
+            for (uint32_t c = 0; c < 8; ++c) acc[r][c] = max(acc[r][c], min(a[r], b[c]));
+    }
+    if (c0 + 8 * tx - kb >= T) {  // columns of the pivot block stay untouched
+#pragma unroll
+        for (uint32_t r = 0; r < RP; ++r)
+#pragma unroll
+            for (uint32_t c = 0; c < 8; ++c) W[(size_t)(kb + RP * ty + r) * ldw + c0 + 8 * tx + c] = (E)acc[r][c];
+    }
+}
+
+// lane: rows 8 ty .. 8 ty + 7 of the chunk, columns (T/16) tx ..
+template <typename E, uint32_t T>
+__device__ __forceinline__ void col_chunk_product(E* __restrict__ W, size_t ldw, uint32_t kb, uint32_t c0,
+                                                  const uint32_t (*P)[T + 1], const uint32_t* S) {
+    constexpr uint32_t RP = T / 16;
+    const uint32_t t = threadIdx.x, ty = t >> 4, tx = t & 15;
+    uint32_t acc[8][RP];
+#pragma unroll
+    for (uint32_t r = 0; r < 8; ++r)
+#pragma unroll
+        for (uint32_t c = 0; c < RP; ++c) acc[r][c] = S[(8 * ty + r) * (T + 1) + RP * tx + c];
+#pragma unroll 4
+    for (uint32_t k = 0; k < T; ++k) {
+        uint32_t a[8], b[RP];
+#pragma unroll
+        for (uint32_t r = 0; r < 8; ++r) a[r] = S[(8 * ty + r) * (T + 1) + k];
+#pragma unroll
+        for (uint32_t c = 0; c < RP; ++c) b[c] = P[k][RP * tx + c];
+#pragma unroll
+        for (uint32_t r = 0; r < 8; ++r)
+#pragma unroll
+            for (uint32_t c = 0; c < RP; ++c) acc[r][c] = max(acc[r][c], min(a[r], b[c]));
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < 8; ++r) {
+        const uint32_t i = c0 + 8 * ty + r;
+        if (i - kb < T) continue;  // rows of the pivot block stay untouched
+#pragma unroll
+        for (uint32_t c = 0; c < RP; ++c) W[(size_t)i * ldw + kb + RP * tx + c] = (E)acc[r][c];
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Independent tiles, 16-bit.  Workgroup = 256 lanes = 128 x 128 macro tile; lane (ty, tx)
+// owns rows ty + 16 r (r < 8) and columns 8 tx .. 8 tx + 7, i.e. 8 x 4 packed u16 pairs.
+// Rows are interleaved by 16 so the four row addresses a wave reads from the A strip fall
+// in different LDS banks; the B strip row a wave reads is 256 contiguous bytes.
+// Per pivot step: 32 v_pk_min_u16 + 32 v_pk_max_u16 for 64 cells, a[i][k] broadcast to
+// both halves by op_sel, and two ds_read_b128 per lane per step.
+
+template <uint32_t T>
+constexpr uint32_t indep_u16_lds_elems() { return kMacro * (T + 8) + T * kMacro; }
+
+// C(I0.., J0..) = max(C, A (x) B) on one 128 x 128 macro tile; As/Bs are LDS buffers of
+// indep_u16_lds_elems<T>() u16 in total.  Starts with a barrier (LDS reuse between calls).
+template <uint32_t T>
+__device__ __forceinline__ void indep_tile_u16(uint16_t* __restrict__ W, size_t ldw, uint32_t kb, uint32_t I0,
+                                               uint32_t J0, uint16_t* __restrict__ lds) {
+    constexpr uint32_t AP = T + 8;  // A strip pitch (u16), keeps 16-byte alignment
+    uint16_t* As = lds;
+    uint16_t* Bs = lds + kMacro * AP;
+    const uint32_t t = threadIdx.x, tx = t & 15, ty = t >> 4;
+    __syncthreads();
+
+    uint4 c[8];
+    const uint16_t* cp = W + (size_t)(I0 + ty) * ldw + J0 + tx * 8;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) c[r] = *reinterpret_cast<const uint4*>(cp + (size_t)(16 * r) * ldw);
+
+    for (uint32_t q = t; q < kMacro * (T / 8); q += kThreads) {
+        uint32_t row = q / (T / 8), ch = q % (T / 8);
+        *reinterpret_cast<uint4*>(As + row * AP + ch * 8) =
+            *reinterpret_cast<const uint4*>(W + (size_t)(I0 + row) * ldw + kb + ch * 8);
+    }
+    for (uint32_t q = t; q < T * 16; q += kThreads) {
+        uint32_t row = q >> 4, ch = q & 15;
+        *reinterpret_cast<uint4*>(Bs + row * kMacro + ch * 8) =
+            *reinterpret_cast<const uint4*>(W + (size_t)(kb + row) * ldw + J0 + ch * 8);
+    }
+    __syncthreads();
+
+    u16x2 acc[8][4];
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[r][q] = as_u16x2(word(c[r], q));
+
+#pragma unroll 1
+    for (uint32_t k0 = 0; k0 < T; k0 += 8) {
+        uint4 a4[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+            a4[r] = *reinterpret_cast<const uint4*>(As + (ty + 16 * r) * AP + k0);
+#pragma unroll
+        for (int kk = 0; kk < 8; ++kk) {
+            const uint4 b4 = *reinterpret_cast<const uint4*>(Bs + (k0 + kk) * kMacro + tx * 8);
+            const u16x2 b[4] = {as_u16x2(b4.x), as_u16x2(b4.y), as_u16x2(b4.z), as_u16x2(b4.w)};
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                const u16x2 a2 = as_u16x2(word(a4[r], kk >> 1));
+                const u16x2 av = (kk & 1) ? a2.yy : a2.xx;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[r][q] = pk_max(acc[r][q], pk_min(av, b[q]));
+            }
+        }
+    }
+
+    if (J0 + tx * 8 - kb < T) return;  // these columns belong to the pivot block
+    uint16_t* op = W + (size_t)(I0 + ty) * ldw + J0 + tx * 8;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        if (I0 + ty + 16 * r - kb < T) continue;  // pivot rows are final already
+        uint4 v;
+        v.x = as_u32(acc[r][0]);
+        v.y = as_u32(acc[r][1]);
+        v.z = as_u32(acc[r][2]);
+        v.w = as_u32(acc[r][3]);
+        *reinterpret_cast<uint4*>(op + (size_t)(16 * r) * ldw) = v;
+    }
+}
+
+// Independent tiles, 32-bit.  Lane (ty, tx) owns rows ty + 16 r (r < 8) and columns
+// 4 tx .. 4 tx + 3 and 64 + 4 tx .. 64 + 4 tx + 3 (each half is 256 contiguous bytes per
+// row across 16 lanes, so B-strip reads and global accesses stay conflict-free and
+// coalesced).  Two pivot steps fold into one v_max3_u32: 1.5 VALU per cell update.
+
+template <uint32_t T>
+constexpr uint32_t indep_u32_lds_elems() { return kMacro * (T + 4) + T * kMacro; }
+
+template <uint32_t T>
+__device__ __forceinline__ void indep_tile_u32(uint32_t* __restrict__ W, size_t ldw, uint32_t kb, uint32_t I0,
+                                               uint32_t J0, uint32_t* __restrict__ lds) {
+    constexpr uint32_t AP = T + 4;  // A strip pitch (u32)
+    uint32_t* As = lds;
+    uint32_t* Bs = lds + kMacro * AP;
+    const uint32_t t = threadIdx.x, tx = t & 15, ty = t >> 4;
+    __syncthreads();
+
+    uint4 c[8][2];
+    const uint32_t* cp = W + (size_t)(I0 + ty) * ldw + J0 + tx * 4;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        c[r][0] = *reinterpret_cast<const uint4*>(cp + (size_t)(16 * r) * ldw);
+        c[r][1] = *reinterpret_cast<const uint4*>(cp + (size_t)(16 * r) * ldw + 64);
+    }
+    for (uint32_t q = t; q < kMacro * (T / 4); q += kThreads) {
+        uint32_t row = q / (T / 4), ch = q % (T / 4);
+        *reinterpret_cast<uint4*>(As + row * AP + ch * 4) =
+            *reinterpret_cast<const uint4*>(W + (size_t)(I0 + row) * ldw + kb + ch * 4);
+    }
+    for (uint32_t q = t; q < T * 32; q += kThreads) {
+        uint32_t row = q >> 5, ch = q & 31;
+        *reinterpret_cast<uint4*>(Bs + row * kMacro + ch * 4) =
+            *reinterpret_cast<const uint4*>(W + (size_t)(kb + row) * ldw + J0 + ch * 4);
+    }
+    __syncthreads();
+
+    uint32_t acc[8][8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            acc[r][q] = word(c[r][0], q);
+            acc[r][4 + q] = word(c[r][1], q);
+        }
+
+#pragma unroll 1
+    for (uint32_t k0 = 0; k0 < T; k0 += 4) {
+        uint4 a4[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+            a4[r] = *reinterpret_cast<const uint4*>(As + (ty + 16 * r) * AP + k0);
+#pragma unroll
+        for (int kk = 0; kk < 4; kk += 2) {
+            const uint32_t* b0p = Bs + (k0 + kk) * kMacro + tx * 4;
+            const uint32_t* b1p = b0p + kMacro;
+            const uint4 b00 = *reinterpret_cast<const uint4*>(b0p);
+            const uint4 b01 = *reinterpret_cast<const uint4*>(b0p + 64);
+            const uint4 b10 = *reinterpret_cast<const uint4*>(b1p);
+            const uint4 b11 = *reinterpret_cast<const uint4*>(b1p + 64);
+            const uint32_t b0[8] = {b00.x, b00.y, b00.z, b00.w, b01.x, b01.y, b01.z, b01.w};
+            const uint32_t b1[8] = {b10.x, b10.y, b10.z, b10.w, b11.x, b11.y, b11.z, b11.w};
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                const uint32_t a0 = word(a4[r], kk), a1 = word(a4[r], kk + 1);
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    acc[r][q] = max(max(acc[r][q], min(a0, b0[q])), min(a1, b1[q]));
+            }
+        }
+    }
+
+    const bool lo_pivot = J0 + tx * 4 - kb < T, hi_pivot = J0 + 64 + tx * 4 - kb < T;
+    uint32_t* op = W + (size_t)(I0 + ty) * ldw + J0 + tx * 4;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        if (I0 + ty + 16 * r - kb < T) continue;
+        if (!lo_pivot)
+            *reinterpret_cast<uint4*>(op + (size_t)(16 * r) * ldw) =
+                make_uint4(acc[r][0], acc[r][1], acc[r][2], acc[r][3]);
+        if (!hi_pivot)
+            *reinterpret_cast<uint4*>(op + (size_t)(16 * r) * ldw + 64) =
+                make_uint4(acc[r][4], acc[r][5], acc[r][6], acc[r][7]);
+    }
+}
+
+}  // namespace se

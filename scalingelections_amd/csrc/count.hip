@@ -15,75 +15,26 @@
 
 #include <cstdint>
 
+#include "count_tiles.h"
 #include "se_internal.h"
 
 namespace se {
 
 namespace {
 
-constexpr uint32_t kCT = 128;     // tile side
-constexpr uint32_t kVC = 32;      // voters staged per LDS round
-constexpr uint32_t kCThreads = 256;
-
-// Lane (ty, tx) owns rows {4 ty + a, 64 + 4 ty + a} and columns {4 tx + b, 64 + 4 tx + b}
-// (a, b < 4): each half-row a wave reads from LDS is 256 contiguous bytes.
+// Lane (ty, tx) layout: see count_tile.  Linear block id -> (I, J), I <= J, row-major over
+// the upper triangle of 128 x 128 tiles.
 __global__ __launch_bounds__(256) void count_kernel(const int32_t* __restrict__ pos, uint32_t m, uint32_t n,
                                                     size_t ldp, uint32_t* __restrict__ d, size_t ldd,
                                                     uint32_t nb) {
     __shared__ __align__(16) int32_t Pi[kVC][kCT];
     __shared__ __align__(16) int32_t Pj[kVC][kCT];
-    // linear block id -> (I, J) with I <= J, row-major over the upper triangle
     uint32_t t = blockIdx.x, I = 0;
     while (t >= nb - I) {
         t -= nb - I;
         ++I;
     }
-    const uint32_t J = I + t;
-    const uint32_t I0 = I * kCT, J0 = J * kCT;
-    const uint32_t tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-
-    uint32_t acc[8][8];
-#pragma unroll
-    for (int a = 0; a < 8; ++a)
-#pragma unroll
-        for (int b = 0; b < 8; ++b) acc[a][b] = 0;
-
-    for (uint32_t v0 = 0; v0 < m; v0 += kVC) {
-        __syncthreads();
-        for (uint32_t q = threadIdx.x; q < kVC * kCT; q += kCThreads) {
-            uint32_t v = q / kCT, c = q % kCT;
-            bool ok = v0 + v < m;
-            Pi[v][c] = (ok && I0 + c < n) ? pos[(size_t)(v0 + v) * ldp + I0 + c] : 0;
-            Pj[v][c] = (ok && J0 + c < n) ? pos[(size_t)(v0 + v) * ldp + J0 + c] : 0;
-        }
-        __syncthreads();
-        const uint32_t vend = min(kVC, m - v0);
-        for (uint32_t v = 0; v < vend; ++v) {
-            const int4 i0 = *reinterpret_cast<const int4*>(&Pi[v][4 * ty]);
-            const int4 i1 = *reinterpret_cast<const int4*>(&Pi[v][64 + 4 * ty]);
-            const int4 j0 = *reinterpret_cast<const int4*>(&Pj[v][4 * tx]);
-            const int4 j1 = *reinterpret_cast<const int4*>(&Pj[v][64 + 4 * tx]);
-            const int32_t pi[8] = {i0.x, i0.y, i0.z, i0.w, i1.x, i1.y, i1.z, i1.w};
-            const int32_t pj[8] = {j0.x, j0.y, j0.z, j0.w, j1.x, j1.y, j1.z, j1.w};
-#pragma unroll
-            for (int a = 0; a < 8; ++a)
-#pragma unroll
-                for (int b = 0; b < 8; ++b) acc[a][b] += (uint32_t)(pi[a] < pj[b]);
-        }
-    }
-
-#pragma unroll
-    for (int a = 0; a < 8; ++a) {
-        const uint32_t i = I0 + (a < 4 ? 4 * ty + a : 64 + 4 * ty + (a - 4));
-        if (i >= n) continue;
-#pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            const uint32_t j = J0 + (b < 4 ? 4 * tx + b : 64 + 4 * tx + (b - 4));
-            if (j >= n) continue;
-            d[(size_t)i * ldd + j] = acc[a][b];
-            if (I != J) d[(size_t)j * ldd + i] = m - acc[a][b];
-        }
-    }
+    count_tile(pos, m, n, ldp, d, ldd, I, I + t, Pi, Pj);
 }
 
 __global__ void zero_matrix_kernel(uint32_t* d, uint32_t n, size_t ldd) {

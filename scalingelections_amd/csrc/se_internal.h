@@ -30,6 +30,8 @@ enum Slot : int {
     kSlotSmall = 3,    // scalars: max(p0), counters
     kSlotPivot = 4,    // pivot-tile hand-off between kernels
     kSlotAux = 5,      // positions / wins staging
+    kSlotBatchD = 6,   // batched preference matrices
+    kSlotBatchMeta = 7,  // batch metadata / per-launch wins
     kSlotCount = 8
 };
 

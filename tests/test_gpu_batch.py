@@ -1,0 +1,75 @@
+"""Batched mixed-size elections: every election's strongest paths and ranking from the
+batched launches equal the per-launch form's and the oracle's, bit for bit."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+EDGE_C = [1, 2, 3, 7, 8, 9, 31, 32, 33, 63, 64, 65, 127, 128, 129, 200, 255, 256, 257, 300, 511, 512, 513, 700]
+
+
+def _oracle_election(oracle_mod, pos, C):
+    m = pos.shape[0]
+    if m == 0:
+        d = np.zeros((C, C), np.uint32)
+    else:
+        rankings = [np.argsort(row, kind="stable") for row in pos]
+        d = oracle_mod.build_prefs(rankings)
+        assert d.shape == (C, C)
+    p = oracle_mod.closure_tiled(d, 16)
+    return d, p, oracle_mod.ranking(p)
+
+
+def _check(oracle_mod, w, label, sample=None):
+    from scalingelections_amd import batch
+
+    db = batch.DeviceBatch(w)
+    db.run(per_launch=False)
+    pb, rb = db.results()
+    db.run(per_launch=True)
+    pl, rl = db.results()
+    idx = range(w.elections) if sample is None else sample
+    for e in range(w.elections):
+        assert np.array_equal(pb[e], pl[e]), (label, "paths batched vs per-launch", e, int(w.C[e]))
+        assert np.array_equal(rb[e], rl[e]), (label, "ranking batched vs per-launch", e, int(w.C[e]))
+    for e in idx:
+        _, p, r = _oracle_election(oracle_mod, w.election_positions(e), int(w.C[e]))
+        assert np.array_equal(pb[e], p), (label, "paths vs oracle", e, int(w.C[e]))
+        assert np.array_equal(rb[e].astype(np.int64), r), (label, "ranking vs oracle", e, int(w.C[e]))
+
+
+def test_batch_edge_sizes(gpu_lib, oracle_mod):
+    from scalingelections_amd import batch
+
+    rng = np.random.default_rng(0)
+    C = np.array(EDGE_C, dtype=np.uint32)
+    m = rng.integers(1, 120, C.shape[0]).astype(np.uint32)
+    phi = rng.uniform(0.5, 1.0, C.shape[0])
+    print("batch edge sizes", C.tolist(), flush=True)
+    _check(oracle_mod, batch.make_workload(C, m, phi, seed=11), "edge")
+
+
+def test_batch_zero_and_single_voter(gpu_lib, oracle_mod):
+    from scalingelections_amd import batch
+
+    C = np.array([5, 70, 1, 130, 2], dtype=np.uint32)
+    m = np.array([0, 1, 0, 1, 3], dtype=np.uint32)
+    _check(oracle_mod, batch.make_workload(C, m, np.ones(5), seed=12), "tiny")
+
+
+def test_batch_mixed_sample(gpu_lib, oracle_mod):
+    """200 elections from the mixed-size distribution (smaller voter counts); all compared
+    batched vs per-launch, 40 against the oracle."""
+    from scalingelections_amd import batch
+
+    w = batch.mixed_size_workload(200, seed=13, m_range=(20, 400))
+    _check(oracle_mod, w, "mixed", sample=range(0, 200, 5))
+
+
+def test_batch_rejects_oversize_election(gpu_lib):
+    from scalingelections_amd import batch
+
+    w = batch.make_workload(np.array([1025], np.uint32), np.array([1], np.uint32), np.ones(1), seed=1)
+    db = batch.DeviceBatch(w)
+    with pytest.raises(RuntimeError, match="1024 candidates"):
+        db.run()

@@ -225,3 +225,40 @@ def test_product_does_not_import_oracle():
                 text = open(os.path.join(dirpath, f)).read()
                 assert "import oracle" not in text and "from oracle" not in text, f
                 assert "liboracle" not in text and "libref" not in text, f
+
+
+def test_mallows_generator_is_deterministic_and_permutations():
+    from scalingelections_amd import batch
+
+    C = np.array([1, 5, 64, 65, 130], np.uint32)
+    m = np.array([3, 50, 20, 7, 4], np.uint32)
+    phi = np.array([1.0, 0.5, 0.9, 0.7, 1.0])
+    ref = np.concatenate([np.random.default_rng(i).permutation(int(c)) for i, c in enumerate(C)]).astype(np.int32)
+    a = batch.mallows_positions(C, m, phi, ref, seed=9, threads=1)
+    b = batch.mallows_positions(C, m, phi, ref, seed=9, threads=7)
+    assert np.array_equal(a, b)
+    o = 0
+    for c, mm in zip(C, m):
+        blk = a[o:o + int(c) * int(mm)].reshape(int(mm), int(c))
+        assert (np.sort(blk, axis=1) == np.arange(int(c))).all()
+        o += int(c) * int(mm)
+
+
+@pytest.mark.parametrize("phi", [0.5, 1.0])
+def test_mallows_generator_matches_closed_form(phi):
+    """Mallows with dispersion phi: P(ballot = reference order) = 1 / prod_k Z_k with
+    Z_k = sum_{t<=k} phi^t, and the ballot's inversion count has mean sum_k E[off_k]."""
+    from scalingelections_amd import batch
+
+    C, M = 5, 40000
+    ref = np.array([3, 0, 4, 1, 2], np.int32)
+    pos = batch.mallows_positions(np.array([C], np.uint32), np.array([M], np.uint32), np.array([phi]), ref, seed=5)
+    pos = pos.reshape(M, C)
+    rank_of_ref = pos[:, ref]  # place of the k-th reference item
+    Z = [sum(phi ** t for t in range(k + 1)) for k in range(C)]
+    p_id = 1.0 / np.prod(Z)
+    got = float((rank_of_ref == np.arange(C)).all(axis=1).mean())
+    assert abs(got - p_id) < 5 * np.sqrt(p_id * (1 - p_id) / M)
+    inv = sum((rank_of_ref[:, i] > rank_of_ref[:, j]).astype(np.int64) for i in range(C) for j in range(i + 1, C))
+    mean_off = sum(sum(t * phi ** t for t in range(k + 1)) / Z[k] for k in range(C))
+    assert abs(inv.mean() - mean_off) < 0.05
