@@ -69,48 +69,7 @@ __global__ __launch_bounds__(256) void batch_count_kernel(const int32_t* __restr
         count_tile(pos, m, C, C, d, C, it.y, it.z, Pi, Pj);
         return;
     }
-    // voter-parallel small election
-    const uint32_t t = threadIdx.x;
-    const uint32_t nblk = (C + 7) / 8, blocks = nblk * nblk, G = kCThreads / blocks;
-    const uint32_t b = t % blocks, g = t / blocks;
-    const uint32_t bi = b / nblk, bj = b % nblk;
-    uint32_t acc[8][8];
-#pragma unroll
-    for (int a = 0; a < 8; ++a)
-#pragma unroll
-        for (int c = 0; c < 8; ++c) acc[a][c] = 0;
-    if (g < G) {
-        for (uint32_t v = g; v < m; v += G) {
-            const int32_t* row = pos + (size_t)v * C;
-            int32_t pi[8], pj[8];
-#pragma unroll
-            for (int a = 0; a < 8; ++a) {
-                const uint32_t ci = 8 * bi + a, cj = 8 * bj + a;
-                pi[a] = ci < C ? row[ci] : 0;
-                pj[a] = cj < C ? row[cj] : 0;
-            }
-#pragma unroll
-            for (int a = 0; a < 8; ++a)
-#pragma unroll
-                for (int c = 0; c < 8; ++c) acc[a][c] += (uint32_t)(pi[a] < pj[c]);
-        }
-    }
-    // sum the G partial blocks, 8 counters at a time, through LDS (Pi reused: 256 x 8 words)
-    int32_t* red = &Pi[0][0];
-#pragma unroll
-    for (int a = 0; a < 8; ++a) {
-        __syncthreads();
-#pragma unroll
-        for (int c = 0; c < 8; ++c) red[t * 8 + c] = (int32_t)acc[a][c];
-        __syncthreads();
-        for (uint32_t q = t; q < blocks * 8; q += kCThreads) {
-            const uint32_t bb = q / 8, c = q % 8;
-            uint32_t sum = 0;
-            for (uint32_t gg = 0; gg < G; ++gg) sum += (uint32_t)red[(gg * blocks + bb) * 8 + c];
-            const uint32_t i = 8 * (bb / nblk) + a, j = 8 * (bb % nblk) + c;
-            if (i < C && j < C) d[(size_t)i * C + j] = sum;
-        }
-    }
+    count_small(pos, m, C, C, d, C, &Pi[0][0]);
 }
 
 // ------------------------------------------------------------------------- closure

@@ -71,4 +71,51 @@ __device__ __forceinline__ void count_tile(const int32_t* __restrict__ pos, uint
     }
 }
 
+// Whole election with n <= 128 candidates, voter-parallel: the n x n counters are cut
+// into 8 x 8 blocks (at most 256) and the 256 lanes into G = 256 / blocks voter groups;
+// group g counts voters g, g + G, ...; partial blocks are summed through LDS (`red`, at
+// least 256 * 8 words) and written as the full matrix (both triangles, zero diagonal).
+__device__ __forceinline__ void count_small(const int32_t* __restrict__ pos, uint32_t m, uint32_t n, size_t ldp,
+                                            uint32_t* __restrict__ d, size_t ldd, int32_t* red) {
+    const uint32_t t = threadIdx.x;
+    const uint32_t nblk = (n + 7) / 8, blocks = nblk * nblk, G = kCThreads / blocks;
+    const uint32_t b = t % blocks, g = t / blocks;
+    const uint32_t bi = b / nblk, bj = b % nblk;
+    uint32_t acc[8][8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[a][c] = 0;
+    if (g < G) {
+        for (uint32_t v = g; v < m; v += G) {
+            const int32_t* row = pos + (size_t)v * ldp;
+            int32_t pi[8], pj[8];
+#pragma unroll
+            for (int a = 0; a < 8; ++a) {
+                const uint32_t ci = 8 * bi + a, cj = 8 * bj + a;
+                pi[a] = ci < n ? row[ci] : 0;
+                pj[a] = cj < n ? row[cj] : 0;
+            }
+#pragma unroll
+            for (int a = 0; a < 8; ++a)
+#pragma unroll
+                for (int c = 0; c < 8; ++c) acc[a][c] += (uint32_t)(pi[a] < pj[c]);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < 8; ++c) red[t * 8 + c] = (int32_t)acc[a][c];
+        __syncthreads();
+        for (uint32_t q = t; q < blocks * 8; q += kCThreads) {
+            const uint32_t bb = q / 8, c = q % 8;
+            uint32_t sum = 0;
+            for (uint32_t gg = 0; gg < G; ++gg) sum += (uint32_t)red[(gg * blocks + bb) * 8 + c];
+            const uint32_t i = 8 * (bb / nblk) + a, j = 8 * (bb % nblk) + c;
+            if (i < n && j < n) d[(size_t)i * ldd + j] = sum;
+        }
+    }
+}
+
 }  // namespace se

@@ -32,7 +32,8 @@ enum Slot : int {
     kSlotAux = 5,      // positions / wins staging
     kSlotBatchD = 6,   // batched preference matrices
     kSlotBatchMeta = 7,  // batch metadata / per-launch wins
-    kSlotCount = 8
+    kSlotCountPart = 8,  // per-slice partial preference matrices
+    kSlotCount = 10
 };
 
 struct DeviceContext {
