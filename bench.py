@@ -1,23 +1,32 @@
-"""Benchmark: strongest-path closure throughput on one MI355X.
+"""Benchmark: strongest-path closure throughput on one MI355X, plus the batch and serving
+figures of the same metric line.
 
-Workload (one step): the full device pipeline of one 32768-candidate election --
+Headline workload (one step): the full device pipeline of one 32768-candidate election --
 strongest-path init (p0) + blocked max-min Floyd-Warshall + output -- on the synthesized
 preference matrix (for i < j, d[i][j] ~ Binomial(65535, clip(0.5 + 0.02 z, 0, 1)),
 d[j][i] = 65535 - d[i][j], 5% forced ties; seed 3).  The input is resident in HBM before
 timing starts.  value = n^3 / step time (TCells/s, the reference's cells^3/s convention).
 
-Also reported, all from this run:
-  * roofline of the dominant kernel (the independent-tile kernel): algorithmic integer ops
-    per launch (2 per cell update: one min, one max) / its average launch duration from
-    HIP events recorded around every launch during the timed steps, against the packed
-    16-bit VALU peak;
-  * the end-to-end host-to-host call (the reference's timing convention: input upload,
-    everything above, result download) and its bit-exact check against the committed
-    oracle fixture when present;
-  * cpu_baseline: the CPU oracle (the tiled restatement of the reference's Numba kernel,
-    T = 16) on the same matrix for a bounded number of pivot rounds, on the host's cores.
+Also reported, all measured in this run:
+  * roofline of the dominant kernel (independent tiles): algorithmic integer ops per launch
+    (2 per cell update) / its average launch duration from HIP events recorded around every
+    launch on the library stream during the timed steps, against the packed 16-bit VALU peak;
+  * end_to_end: the reference's convention (host upload + closure + download) with a
+    sha256 check of p against the committed oracle fixture;
+  * closure_by_size: device TCells/s at 4096, 8192 and 16384 candidates;
+  * counter: 4096 ballots x 4096 candidates (seed 2), pair-votes/s against the 32-bit
+    VALU compare+add peak, and the whole 4096 x 4096 election (count, closure, ranking);
+  * batch: 4096 mixed-size Mallows elections (C log-uniform in [8, 512], m uniform in
+    [100, 10000], seed 4), elections/s batched vs one launch sequence per election, with
+    every election's paths and ranking compared between the two;
+  * serving: Poisson arrivals from that mix plus 1% 8192 x 8192 Mallows elections, offered
+    load set to 60% of the capacity measured above, p50/p99 latency, every result checked
+    against the same election processed alone;
+  * cpu_baseline: the CPU oracle (tiled restatement of the reference's Numba kernel,
+    T = 16) on the 32768 matrix for a bounded number of pivot rounds.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 32768] [--no-cpu] [--no-e2e]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 32768]
+                       [--no-cpu] [--no-e2e] [--no-extra]
 """
 from __future__ import annotations
 
@@ -35,14 +44,19 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 # MI355X: 256 CUs x 64 lanes x 2.4 GHz = 3.93e13 32-bit VALU ops/s; packed 16-bit ops do
-# two per lane (7.86e13/s).  One cell update = 2 integer ops (min, max).
-PEAK_OPS_U16 = 256 * 64 * 2.4e9 * 2
+# two per lane (7.86e13/s).  One cell update = 2 integer ops (min, max); one counted
+# pair-vote = 2 ops (compare, add with carry).
 PEAK_OPS_U32 = 256 * 64 * 2.4e9
+PEAK_OPS_U16 = 2 * PEAK_OPS_U32
 PUBLISHED_TCELLS_32768 = 3.5  # reference README, Mojo on MI355X, end-to-end call
 
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def sha(a) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).astype("<u4", copy=False).tobytes()).hexdigest()
 
 
 def cpu_baseline(d: np.ndarray, budget_s: float = 15.0) -> dict:
@@ -56,21 +70,193 @@ def cpu_baseline(d: np.ndarray, budget_s: float = 15.0) -> dict:
     oracle.closure_tiled(d, T, threads=threads, max_rounds=0)  # p0 init only
     t_init = time.time() - t0
     t0 = time.time()
-    oracle.closure_tiled(d, T, threads=threads, max_rounds=1)
-    t_one = max(1e-3, time.time() - t0 - t_init)
-    rounds = int(max(1, min((n + T - 1) // T, budget_s / t_one)))
+    oracle.closure_tiled(d, T, threads=threads, max_rounds=2)
+    t_two = max(1e-3, time.time() - t0 - t_init)
+    rounds = int(max(1, min((n + T - 1) // T, 2 * budget_s / t_two)))
     t0 = time.time()
     oracle.closure_tiled(d, T, threads=threads, max_rounds=rounds)
     el = max(1e-6, time.time() - t0 - t_init)
     cells = rounds * float(n) * n * T
     return {
-        "value": cells / el / 1e12,
+        "value": round(cells / el / 1e12, 5),
         "unit": "TCells/s",
         "cores": threads,
         "kind": "port",
         "sample": f"first {rounds} of {(n + T - 1) // T} pivot rounds (tile 16) of the same {n}-candidate "
                   f"matrix, {cells:.3e} cell updates in {el:.1f}s (p0 init excluded)",
     }
+
+
+def closure_step_timed(L, _lib, d_ptr, p_ptr, n, flags):
+    t = _lib.se_timing()
+    _lib.check(L.se_closure_device(d_ptr, n, n, p_ptr, n, flags | _lib.SE_FLAG_TIME_KERNELS, None,
+                                   ctypes.byref(t)))
+    return t.as_dict()
+
+
+def closure_by_size(sizes) -> dict:
+    from scalingelections_amd import _lib, devmem, synth
+
+    L = _lib.lib()
+    out = {}
+    for n in sizes:
+        d = synth.synthesized_preferences(n, 3)
+        dd = devmem.DeviceBuffer.from_array(d)
+        pd = devmem.DeviceBuffer(d.nbytes)
+        closure_step_timed(L, _lib, dd.ptr, pd.ptr, n, 0)
+        reps = max(2, int(2e11 / n ** 3))
+        devmem.device_synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            _lib.check(L.se_closure_device(dd.ptr, n, n, pd.ptr, n, 0, None, None))
+        devmem.device_synchronize()
+        el = (time.perf_counter() - t0) / reps
+        out[str(n)] = {"ms": round(el * 1e3, 3), "tcells_per_s": round(n ** 3 / el / 1e12, 3)}
+        log(f"closure n={n}: {out[str(n)]}")
+    return out
+
+
+def counter_section() -> dict:
+    from scalingelections_amd import _lib, devmem, synth
+
+    L = _lib.lib()
+    m = n = 4096
+    R = synth.impartial_culture(m, n, 2)
+    pos = synth.positions_from_rankings(R)
+    pd = devmem.DeviceBuffer.from_array(pos)
+    dd = devmem.DeviceBuffer(n * n * 4)
+    p_dev = devmem.DeviceBuffer(n * n * 4)
+    ms = ctypes.c_double(0)
+    for _ in range(2):
+        _lib.check(L.se_count_device(pd.ptr, m, n, dd.ptr, n, None, ctypes.byref(ms)))
+    best = float("inf")
+    for _ in range(5):
+        _lib.check(L.se_count_device(pd.ptr, m, n, dd.ptr, n, None, ctypes.byref(ms)))
+        best = min(best, ms.value)
+    nb = (n + 127) // 128
+    issued = m * (nb * (nb + 1) // 2) * 128 * 128
+    useful = m * n * (n - 1) / 2
+    # whole election: count + closure + wins + ranking, device-resident
+    wins = np.empty(n, np.uint32)
+    rank = np.empty(n, np.uint32)
+    devmem.device_synchronize()
+    t0 = time.perf_counter()
+    _lib.check(L.se_count_device(pd.ptr, m, n, dd.ptr, n, None, None))
+    _lib.check(L.se_closure_device(dd.ptr, n, n, p_dev.ptr, n, 0, None, None))
+    _lib.check(L.se_rank_device(p_dev.ptr, n, n, wins.ctypes.data, rank.ctypes.data, None))
+    el = time.perf_counter() - t0
+    d = dd.download(np.empty((n, n), np.uint32))
+    fx_path = os.path.join(ROOT, "tests", "golden", "c2_4096x4096_seed2.json")
+    parity = "no fixture"
+    if os.path.exists(fx_path):
+        fx = json.load(open(fx_path))
+        p = p_dev.download(np.empty((n, n), np.uint32))
+        ok = sha(d) == fx["d_sha256"] and sha(p) == fx["p_sha256"]
+        parity = "bit-exact (d and p sha256 match the oracle fixture)" if ok else "MISMATCH"
+    res = {
+        "workload": "4096 impartial-culture ballots x 4096 candidates, seed 2, positions resident in HBM",
+        "count_kernel_ms": round(best, 3),
+        "pair_votes_per_s": round(useful / (best / 1e3), 1),
+        "roofline": {"bound": "valu", "achieved": round(2 * issued / (best / 1e3) / 1e12, 3),
+                     "peak": round(PEAK_OPS_U32 / 1e12, 2), "unit": "Tops/s (integer)",
+                     "frac": round(2 * issued / (best / 1e3) / PEAK_OPS_U32, 4)},
+        "election_ms": round(el * 1e3, 2),
+        "election_tcells_per_s": round(n ** 3 / el / 1e12, 3),
+        "parity": parity,
+    }
+    log(f"counter {res}")
+    return res
+
+
+def batch_and_serving(do_serve: bool) -> tuple:
+    from scalingelections_amd import batch, devmem, serve
+
+    t0 = time.time()
+    w = batch.mixed_size_workload(4096, seed=4)
+    log(f"mixed batch generated in {time.time() - t0:.1f}s ({w.positions.size:.3e} positions)")
+    db = batch.DeviceBatch(w)
+    db.run()
+    devmem.device_synchronize()
+    t0 = time.perf_counter()
+    tm = db.run()
+    devmem.device_synchronize()
+    t_batch = time.perf_counter() - t0
+    Pb, Rb = db.raw()
+    t0 = time.perf_counter()
+    db.run(per_launch=True)
+    devmem.device_synchronize()
+    t_plain = time.perf_counter() - t0
+    Pl, Rl = db.raw()
+    po, ro = w.paths_offsets(), w.rank_offsets()
+    mism = sum(1 for e in range(w.elections)
+               if not (np.array_equal(Pb[po[e]:po[e + 1]], Pl[po[e]:po[e + 1]])
+                       and np.array_equal(Rb[ro[e]:ro[e + 1]], Rl[ro[e]:ro[e + 1]])))
+    bres = {
+        "workload": "4096 elections, C log-uniform [8,512], m uniform [100,10000], Mallows phi U(0.5,1), seed 4; "
+                    "positions resident in HBM",
+        "batched_elections_per_s": round(w.elections / t_batch, 1),
+        "per_launch_elections_per_s": round(w.elections / t_plain, 1),
+        "speedup": round(t_plain / t_batch, 2),
+        "batched_ms": round(t_batch * 1e3, 2), "count_ms": round(tm["count_ms"], 2),
+        "closure_ms": round(tm["closure_ms"], 2),
+        "pair_votes_per_s": round(tm["pair_votes"] / t_batch, 1),
+        "closure_tcells_per_s": round(tm["cells"] / (tm["closure_ms"] / 1e3) / 1e12, 3),
+        "mismatched_elections": mism,
+    }
+    log(f"batch {bres}")
+    if not do_serve:
+        return bres, None
+    # ---- serving
+    small_ref = {e: (Rb[ro[e]:ro[e + 1]].copy(), sha(Pb[po[e]:po[e + 1]])) for e in range(w.elections)}
+    del Pb, Pl, Rl
+    t0 = time.time()
+    large = batch.make_workload(np.array([8192, 8192], np.uint32), np.array([8192, 8192], np.uint32),
+                                np.array([0.9, 1.0]), seed=5)
+    log(f"large pool generated in {time.time() - t0:.1f}s")
+    large_ref = {}
+    t_large = []
+    for e in range(large.elections):
+        t0 = time.perf_counter()
+        large_ref[e] = serve.solo_result(large, e)
+        t_large.append(time.perf_counter() - t0)
+    t_large_s = min(t_large)
+    small_rate = w.elections / t_batch
+    capacity = 1.0 / (0.99 / small_rate + 0.01 * t_large_s)
+    rate = 0.6 * capacity
+    count = int(rate * 4.0)
+    arr = serve.poisson_schedule(rate, count, seed=5)
+    rng = np.random.default_rng(5)
+    reqs = []
+    for k in range(count):
+        if rng.random() < 0.01:
+            reqs.append(serve.Request(k, float(arr[k]), large, int(rng.integers(large.elections))))
+        else:
+            reqs.append(serve.Request(k, float(arr[k]), w, int(rng.integers(w.elections))))
+    srv = serve.Server(large_slots=2, large_max_c=8192, large_max_m=8192)
+    try:
+        res = srv.run(reqs)
+    finally:
+        srv.close()
+    bad = 0
+    for r in res.requests:
+        ref = large_ref[r.index] if r.large else small_ref[r.index]
+        if not (np.array_equal(r.ranking, ref[0]) and r.paths_sha256 == ref[1]):
+            bad += 1
+    s = res.summary()
+    sres = {
+        "workload": "Poisson arrivals; sizes from the batch mix plus 1% 8192 x 8192 Mallows elections",
+        "offered_req_per_s": round(rate, 1),
+        "capacity_estimate_req_per_s": round(capacity, 1),
+        "note": "offered load = 60% of the capacity estimated from the batched rate and one 8192 x 8192 "
+                "election alone; 20k req/s would exceed capacity and only measure queue growth",
+        "completed_req_per_s": s["completed_per_s"],
+        "requests": s["requests"], "batches": s["batches"], "mean_batch": s["mean_batch"],
+        "latency_ms": {k: s[k] for k in ("all", "small", "large") if k in s},
+        "large_solo_ms": round(t_large_s * 1e3, 1),
+        "mismatched_requests": bad,
+    }
+    log(f"serving {sres}")
+    return bres, sres
 
 
 def main() -> int:
@@ -82,11 +268,13 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-e2e", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the size sweep, counter, batch and serving")
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: the closure of one election runs on one device "
-              "(multi-GPU sharding is a design in DESIGN.md, not part of this build)", file=sys.stderr)
+        print("bench.py measures one GPU: this build runs on one device; how the path would shard over "
+              "several GPUs is described in DESIGN.md", file=sys.stderr)
         return 2
 
     from scalingelections_amd import _lib, api, devmem, synth
@@ -101,13 +289,8 @@ def main() -> int:
     p_dev = devmem.DeviceBuffer(d.nbytes)
     L = _lib.lib()
 
-    def step(flags: int, timing=None):
-        rc = L.se_closure_device(d_dev.ptr, n, n, p_dev.ptr, n, flags, None,
-                                 ctypes.byref(timing) if timing is not None else None)
-        _lib.check(rc)
-
     for i in range(args.warmup):
-        step(0)
+        _lib.check(L.se_closure_device(d_dev.ptr, n, n, p_dev.ptr, n, 0, None, None))
         devmem.device_synchronize()
         log(f"warmup {i + 1}/{args.warmup} done")
 
@@ -115,10 +298,9 @@ def main() -> int:
     devmem.device_synchronize()
     t_start = time.perf_counter()
     for i in range(args.steps):
-        t = _lib.se_timing()
-        step(_lib.SE_FLAG_TIME_KERNELS, t)  # synchronises on its last event
-        timings.append(t.as_dict())
-        log(f"step {i + 1}/{args.steps}: {t.total_ms:.1f} ms on device")
+        t = closure_step_timed(L, _lib, d_dev.ptr, p_dev.ptr, n, 0)  # synchronises on its last event
+        timings.append(t)
+        log(f"step {i + 1}/{args.steps}: {t['total_ms']:.1f} ms on device")
     devmem.device_synchronize()
     elapsed = time.perf_counter() - t_start
 
@@ -132,6 +314,7 @@ def main() -> int:
     ops_per_launch = 2.0 * indep_cells / max(1, launches)
     achieved = ops_per_launch / (avg_launch_ms / 1e3) / 1e12 if avg_launch_ms > 0 else 0.0
     peak = (PEAK_OPS_U16 if width == 16 else PEAK_OPS_U32) / 1e12
+    del p_dev
 
     e2e = None
     if not args.no_e2e:
@@ -145,11 +328,16 @@ def main() -> int:
         parity = "no fixture"
         if os.path.exists(fixture):
             want = json.load(open(fixture))["p_sha256"]
-            got = hashlib.sha256(out.astype("<u4", copy=False).tobytes()).hexdigest()
-            parity = "bit-exact (sha256 of p matches the oracle fixture)" if got == want else "MISMATCH"
-        e2e = {"seconds": round(e2e_s, 3), "tcells_per_s": float(n) ** 3 / e2e_s / 1e12, "parity": parity}
+            parity = "bit-exact (sha256 of p matches the oracle fixture)" if sha(out) == want else "MISMATCH"
+        e2e = {"seconds": round(e2e_s, 3), "tcells_per_s": round(float(n) ** 3 / e2e_s / 1e12, 3), "parity": parity}
         log(f"e2e {e2e}")
         del out
+
+    extra = {}
+    if not args.no_extra:
+        extra["closure_by_size"] = closure_by_size([4096, 8192, 16384])
+        extra["counter"] = counter_section()
+        extra["batch"], extra["serving"] = batch_and_serving(not args.no_serve)
 
     cpu = None
     if not args.no_cpu:
@@ -193,9 +381,10 @@ def main() -> int:
             "launches": launches,
         },
         "phases_ms_per_step": {k: round(sum(t[k] for t in timings) / max(1, len(timings)), 2)
-                               for k in ("total_ms", "init_ms", "diag_ms", "panel_ms", "indep_ms", "output_ms")},
+                               for k in ("total_ms", "init_ms", "panel_ms", "indep_ms", "output_ms")},
         "end_to_end": e2e,
         "vs_baseline_end_to_end": round(e2e["tcells_per_s"] / PUBLISHED_TCELLS_32768, 3) if (e2e and n == 32768) else None,
+        **extra,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -15,10 +15,12 @@
  *   se_device_count / se_device_info   log_gpus()            scaling_elections.cu:996-1016
  *   se_reduce_f32                 reduce(float32[])          scaling_elections.cu:1019-1032
  *
- * Threading: calls on one device are serialised by a per-device lock; the library owns
- * its device scratch (a grow-only cache per device) and one stream per device.  The
- * *_device variants take caller-owned device pointers and an optional hipStream_t
- * (passed as void*, NULL = the library's stream for the current device).
+ * Threading: the library keeps one context (stream, lock, grow-only device scratch) per
+ * device, and one more per caller stream passed to a *_device entry point.  Calls sharing
+ * a context are serialised; calls on different caller streams run concurrently and their
+ * work overlaps on the device.  The *_device variants take caller-owned device pointers
+ * and an optional hipStream_t (passed as void*; NULL = the library's stream).
+ * se_stream_release() frees the scratch the library keeps for a caller stream.
  */
 #ifndef SCALING_ELECTIONS_AMD_H
 #define SCALING_ELECTIONS_AMD_H
@@ -168,6 +170,7 @@ float se_reduce_f32(const float *data, size_t count);
 /* Interruption: se_interrupt() sets a flag that running closures check between pivot
  * rounds ("Stopped by signal", scaling_elections.cu:758-759); se_install_sigint() makes
  * SIGINT set it (returns the previous handler state for se_restore_sigint). */
+int se_stream_release(void *stream);
 void se_interrupt(void);
 void se_clear_interrupt(void);
 int se_install_sigint(void);

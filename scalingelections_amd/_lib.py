@@ -38,7 +38,7 @@ EXPORTED = (
     "se_count", "se_count_device", "se_count_rankings",
     "se_rank", "se_rank_device", "se_reduce_f32",
     "se_interrupt", "se_clear_interrupt", "se_install_sigint", "se_restore_sigint",
-    "se_batch_device", "se_synth_mallows",
+    "se_batch_device", "se_synth_mallows", "se_stream_release",
 )
 
 SE_BATCH_PER_LAUNCH = 64
@@ -134,6 +134,8 @@ def _declare(L):
     L.se_batch_device.restype = c.c_int
     L.se_synth_mallows.argtypes = [u32, vp, vp, vp, vp, c.c_uint64, vp, c.c_int]
     L.se_synth_mallows.restype = c.c_int
+    L.se_stream_release.argtypes = [vp]
+    L.se_stream_release.restype = c.c_int
     del i32, i64
 
 

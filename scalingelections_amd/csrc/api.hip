@@ -6,6 +6,7 @@
 #include <atomic>
 #include <csignal>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <numeric>
 #include <string>
@@ -33,6 +34,8 @@ namespace {
 thread_local std::string t_last_error;
 std::mutex g_ctx_mutex;
 std::unique_ptr<DeviceContext> g_ctx[64];
+// contexts of caller streams: own scratch and lock, so calls on different streams overlap
+std::map<std::pair<int, void*>, std::unique_ptr<DeviceContext>> g_stream_ctx;
 struct sigaction g_old_sigint;
 std::mutex g_sig_mutex;
 int g_sigint_depth = 0;
@@ -89,8 +92,32 @@ void* scratch(DeviceContext* ctx, Slot s, size_t bytes) {
     return p;
 }
 
-static hipStream_t pick_stream(DeviceContext* ctx, void* stream) {
-    return stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+// Context for a caller stream (NULL: the library's context of the current device).
+DeviceContext* context_for(void* stream) {
+    if (!stream) return current_context();
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) {
+        set_error("Failed to get current device");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> g(g_ctx_mutex);
+    auto& slot = g_stream_ctx[std::make_pair(dev, stream)];
+    if (!slot) {
+        slot = std::make_unique<DeviceContext>();
+        slot->device = dev;
+        slot->stream = static_cast<hipStream_t>(stream);
+        slot->owns_stream = false;
+    }
+    return slot.get();
+}
+
+static void free_context_scratch(DeviceContext* ctx) {
+    (void)hipStreamSynchronize(ctx->stream);
+    for (int i = 0; i < kSlotCount; ++i) {
+        if (ctx->ptr[i]) (void)hipFree(ctx->ptr[i]);
+        ctx->ptr[i] = nullptr;
+        ctx->cap[i] = 0;
+    }
 }
 
 }  // namespace se
@@ -195,11 +222,10 @@ int se_closure_device(const uint32_t* prefs_dev, uint32_t n, size_t row_stride, 
         return SE_ERR_INVALID;
     }
     g_interrupted.store(0);
-    DeviceContext* ctx = current_context();
+    DeviceContext* ctx = context_for(stream);
     if (!ctx) return SE_ERR_NO_DEVICE;
     std::lock_guard<std::mutex> g(ctx->lock);
-    return closure_device_impl(ctx, prefs_dev, n, row_stride, paths_dev, paths_stride, flags,
-                               pick_stream(ctx, stream), timing);
+    return closure_device_impl(ctx, prefs_dev, n, row_stride, paths_dev, paths_stride, flags, ctx->stream, timing);
 }
 
 int se_closure_cpu(const uint32_t* prefs, uint32_t n, size_t row_stride, uint32_t* paths, uint32_t tile_size,
@@ -283,11 +309,10 @@ int se_count_device(const int32_t* positions_dev, uint32_t m, uint32_t n, uint32
         set_error("invalid positions/preferences device buffers");
         return SE_ERR_INVALID;
     }
-    DeviceContext* ctx = current_context();
+    DeviceContext* ctx = context_for(stream);
     if (!ctx) return SE_ERR_NO_DEVICE;
     std::lock_guard<std::mutex> g(ctx->lock);
-    return count_device_impl(ctx, positions_dev, m, n, prefs_dev, prefs_stride, pick_stream(ctx, stream),
-                             kernel_ms);
+    return count_device_impl(ctx, positions_dev, m, n, prefs_dev, prefs_stride, ctx->stream, kernel_ms);
 }
 
 int se_count_rankings(const int32_t* ids, const int64_t* offsets, uint32_t m, uint32_t n, uint32_t* prefs,
@@ -349,10 +374,10 @@ int se_rank_device(const uint32_t* paths_dev, uint32_t n, size_t row_stride, uin
         set_error("invalid paths/wins buffers");
         return SE_ERR_INVALID;
     }
-    DeviceContext* ctx = current_context();
+    DeviceContext* ctx = context_for(stream);
     if (!ctx) return SE_ERR_NO_DEVICE;
     std::lock_guard<std::mutex> g(ctx->lock);
-    hipStream_t s = pick_stream(ctx, stream);
+    hipStream_t s = ctx->stream;
     uint32_t* w = static_cast<uint32_t*>(scratch(ctx, kSlotSmall, (size_t)n * sizeof(uint32_t)));
     if (!w) return SE_ERR_ALLOC;
     int rc = wins_device_impl(ctx, paths_dev, n, row_stride, w, s);
@@ -409,17 +434,41 @@ int se_batch_device(uint32_t elections, const uint32_t* candidates, const uint32
         set_error("invalid batch buffers");
         return SE_ERR_INVALID;
     }
-    DeviceContext* ctx = current_context();
+    DeviceContext* ctx = context_for(stream);
     if (!ctx) return SE_ERR_NO_DEVICE;
     std::lock_guard<std::mutex> g(ctx->lock);
     return batch_device_impl(ctx, elections, candidates, voters, positions_dev, paths_dev, ranking_dev, flags,
-                             pick_stream(ctx, stream), timing);
+                             ctx->stream, timing);
 }
 
 float se_reduce_f32(const float* data, size_t count) {
     float acc = 0.0f;
     for (size_t i = 0; i < count; ++i) acc = acc + data[i];
     return acc;
+}
+
+int se_stream_release(void* stream) {
+    clear_error();
+    if (!stream) {
+        set_error("stream must not be NULL");
+        return SE_ERR_INVALID;
+    }
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        set_error("Failed to get current device");
+        return SE_ERR_NO_DEVICE;
+    }
+    std::unique_ptr<DeviceContext> ctx;
+    {
+        std::lock_guard<std::mutex> g(g_ctx_mutex);
+        auto it = g_stream_ctx.find(std::make_pair(dev, stream));
+        if (it == g_stream_ctx.end()) return SE_OK;
+        ctx = std::move(it->second);
+        g_stream_ctx.erase(it);
+    }
+    std::lock_guard<std::mutex> g(ctx->lock);
+    free_context_scratch(ctx.get());
+    return SE_OK;
 }
 
 void se_interrupt(void) { g_interrupted.store(1); }

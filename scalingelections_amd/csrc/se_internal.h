@@ -39,6 +39,7 @@ enum Slot : int {
 struct DeviceContext {
     int device = -1;
     hipStream_t stream = nullptr;
+    bool owns_stream = true;
     std::mutex lock;
     void* ptr[kSlotCount] = {};
     size_t cap[kSlotCount] = {};

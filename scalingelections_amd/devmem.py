@@ -32,6 +32,7 @@ def hip():
         H.hipMemcpyAsync.argtypes = [vp, vp, sz, ctypes.c_int, vp]
         H.hipMemsetAsync.argtypes = [vp, ctypes.c_int, sz, vp]
         H.hipStreamCreate.argtypes = [ctypes.POINTER(vp)]
+        H.hipStreamCreateWithFlags.argtypes = [ctypes.POINTER(vp), ctypes.c_uint]
         H.hipStreamDestroy.argtypes = [vp]
         H.hipStreamSynchronize.argtypes = [vp]
         H.hipDeviceSynchronize.argtypes = []
@@ -42,6 +43,7 @@ def hip():
         H.hipEventDestroy.argtypes = [vp]
         H.hipMemGetInfo.argtypes = [ctypes.POINTER(sz), ctypes.POINTER(sz)]
         for f in ("hipMalloc", "hipFree", "hipMemcpy", "hipMemcpyAsync", "hipMemsetAsync", "hipStreamCreate",
+                  "hipStreamCreateWithFlags",
                   "hipStreamDestroy", "hipStreamSynchronize", "hipDeviceSynchronize", "hipEventCreate",
                   "hipEventRecord", "hipEventSynchronize", "hipEventElapsedTime", "hipEventDestroy",
                   "hipMemGetInfo"):
@@ -76,6 +78,18 @@ class DeviceBuffer:
         assert a.nbytes <= self.nbytes
         _ok(hip().hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, hipMemcpyHostToDevice), "hipMemcpy H2D")
 
+    def upload_async(self, a: np.ndarray, stream: "Stream", offset: int = 0) -> None:
+        """Queues a host-to-device copy on `stream`; `a` must stay alive until the stream
+        has passed the copy."""
+        assert a.flags.c_contiguous and offset + a.nbytes <= self.nbytes
+        _ok(hip().hipMemcpyAsync(self.ptr + offset, a.ctypes.data, a.nbytes, hipMemcpyHostToDevice, stream.handle),
+            "hipMemcpyAsync H2D")
+
+    def download_async(self, out: np.ndarray, stream: "Stream", offset: int = 0) -> None:
+        assert out.flags.c_contiguous and offset + out.nbytes <= self.nbytes
+        _ok(hip().hipMemcpyAsync(out.ctypes.data, self.ptr + offset, out.nbytes, hipMemcpyDeviceToHost, stream.handle),
+            "hipMemcpyAsync D2H")
+
     def download(self, out: np.ndarray) -> np.ndarray:
         assert out.flags.c_contiguous and out.nbytes <= self.nbytes
         _ok(hip().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, hipMemcpyDeviceToHost), "hipMemcpy D2H")
@@ -94,9 +108,15 @@ class DeviceBuffer:
 
 
 class Stream:
-    def __init__(self):
+    """A HIP stream at the default priority.  nonblocking=True: no implicit ordering with
+    the null stream, so work on different streams overlaps."""
+
+    def __init__(self, nonblocking: bool = False):
         s = ctypes.c_void_p()
-        _ok(hip().hipStreamCreate(ctypes.byref(s)), "hipStreamCreate")
+        if nonblocking:
+            _ok(hip().hipStreamCreateWithFlags(ctypes.byref(s), 1), "hipStreamCreateWithFlags")
+        else:
+            _ok(hip().hipStreamCreate(ctypes.byref(s)), "hipStreamCreate")
         self.handle = s.value
 
     def synchronize(self) -> None:
@@ -104,6 +124,7 @@ class Stream:
 
     def close(self) -> None:
         if self.handle:
+            _lib.lib().se_stream_release(self.handle)  # scratch the library keeps for this stream
             hip().hipStreamDestroy(self.handle)
             self.handle = None
 
