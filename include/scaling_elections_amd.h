@@ -52,6 +52,8 @@ extern "C" {
                                     /* (scaling_elections.cu:555-558)                     */
 #define SE_FLAG_PIVOT32 16u         /* force pivot depth 32 (default: 64 for 16-bit, 32    */
 #define SE_FLAG_PIVOT64 32u         /* for 32-bit); the result never depends on it         */
+#define SE_FLAG_NO_F16 128u         /* 16-bit path: keep integer min/max even when the     */
+                                    /* values fit the f16-ordered range                   */
 
 /* Per-call measurements, filled when a non-NULL pointer is passed. */
 typedef struct se_timing {
@@ -67,6 +69,9 @@ typedef struct se_timing {
     uint32_t padded_n;        /* matrix side after zero padding                        */
     uint64_t indep_cells;     /* cell updates issued by the independent kernel (all    */
                               /* rounds) = sum over rounds of (N - T)^2 * T            */
+    uint32_t f16_ordered;     /* 1: 16-bit values offset into positive f16 normals and  */
+                              /* closed with f16 minimum/maximum (exact, see closure)  */
+    uint32_t reserved;
 } se_timing;
 
 /* ---- diagnostics -------------------------------------------------------------------- */

@@ -30,6 +30,7 @@ SE_FLAG_TIME_KERNELS = 4
 SE_FLAG_ALLOW_TMA = 8
 SE_FLAG_PIVOT32 = 16
 SE_FLAG_PIVOT64 = 32
+SE_FLAG_NO_F16 = 128
 
 # Every symbol include/scaling_elections_amd.h declares.
 EXPORTED = (
@@ -72,6 +73,8 @@ class se_timing(ctypes.Structure):
         ("width_bits", ctypes.c_uint32),
         ("padded_n", ctypes.c_uint32),
         ("indep_cells", ctypes.c_uint64),
+        ("f16_ordered", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
     def as_dict(self) -> dict:

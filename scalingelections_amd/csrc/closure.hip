@@ -37,11 +37,13 @@ namespace se {
 
 constexpr uint32_t kInitTile = 64;
 
-// max over p0 of the n x n domain -> atomicMax(*out).  Decides the 16-bit eligibility.
+// Range of p0 over the n x n domain: out[0] = max (atomicMax), out[1] = smallest nonzero
+// value (atomicMin, stays 0xFFFFFFFF when p0 is all zero).  Decides the 16-bit eligibility
+// and whether the values fit the f16-ordered range.
 __global__ __launch_bounds__(256) void p0_max_kernel(const uint32_t* __restrict__ d, uint32_t n,
                                                      size_t ld, uint32_t* __restrict__ out) {
     __shared__ uint32_t tr[kInitTile][kInitTile + 1];
-    __shared__ uint32_t red[kThreads / 64];
+    __shared__ uint32_t red[kThreads / 64], redmin[kThreads / 64];
     const uint32_t I0 = blockIdx.y * kInitTile, J0 = blockIdx.x * kInitTile;
     const uint32_t tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     // tr[b][a] = d[J0 + b][I0 + a]
@@ -50,29 +52,41 @@ __global__ __launch_bounds__(256) void p0_max_kernel(const uint32_t* __restrict_
         tr[b][tx] = (r < n && c < n) ? d[(size_t)r * ld + c] : 0u;
     }
     __syncthreads();
-    uint32_t m = 0;
+    uint32_t m = 0, mn = 0xFFFFFFFFu;
     for (uint32_t a = ty; a < kInitTile; a += 4) {
         uint32_t i = I0 + a, j = J0 + tx;
         if (i < n && j < n && i != j) {
             uint32_t v = d[(size_t)i * ld + j];
-            if (v > tr[tx][a]) m = max(m, v);
+            if (v > tr[tx][a]) {  // v > d[j][i] >= 0, so v is a nonzero p0 value
+                m = max(m, v);
+                mn = min(mn, v);
+            }
         }
     }
     // wave reduction then block reduction
-    for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off, 64));
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    for (int off = 32; off > 0; off >>= 1) {
+        m = max(m, (uint32_t)__shfl_xor((int)m, off, 64));
+        mn = min(mn, (uint32_t)__shfl_xor((int)mn, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[threadIdx.x >> 6] = m;
+        redmin[threadIdx.x >> 6] = mn;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t r = max(max(red[0], red[1]), max(red[2], red[3]));
+        uint32_t rn = min(min(redmin[0], redmin[1]), min(redmin[2], redmin[3]));
         if (r) atomicMax(out, r);
+        if (rn != 0xFFFFFFFFu) atomicMin(out + 1, rn);
     }
 }
 
-// Writes p0 into the padded work matrix W (N x N, row pitch ldw, element E).
+// Writes p0 into the padded work matrix W (N x N, row pitch ldw, element E); nonzero
+// values are stored as v + bias (bias = 0, or the f16-ordered offset 0x0400 - min).
 template <typename E>
 __global__ __launch_bounds__(256) void p0_write_kernel(const uint32_t* __restrict__ d, uint32_t n,
                                                        size_t ld, E* __restrict__ W, uint32_t N,
-                                                       size_t ldw) {
+                                                       size_t ldw, uint32_t bias) {
     __shared__ uint32_t tr[kInitTile][kInitTile + 1];
     const uint32_t I0 = blockIdx.y * kInitTile, J0 = blockIdx.x * kInitTile;
     const uint32_t tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
@@ -87,7 +101,7 @@ __global__ __launch_bounds__(256) void p0_write_kernel(const uint32_t* __restric
         uint32_t v = 0;
         if (i < n && j < n && i != j) {
             uint32_t x = d[(size_t)i * ld + j];
-            if (x > tr[tx][a]) v = x;
+            if (x > tr[tx][a]) v = x + bias;
         }
         W[(size_t)i * ldw + j] = (E)v;
     }
@@ -142,10 +156,10 @@ __global__ __launch_bounds__(256) void commit_pivot_kernel(E* __restrict__ W, si
 // ---------------------------------------------------------------------------------------
 // Independent tiles: one 128 x 128 macro tile per workgroup (indep_tile_u16 / _u32).
 
-template <uint32_t T>
+template <uint32_t T, bool F16>
 __global__ __launch_bounds__(256) void indep_u16_kernel(uint16_t* __restrict__ W, size_t ldw, uint32_t kb) {
     __shared__ __align__(16) uint16_t lds[indep_u16_lds_elems<T>()];
-    indep_tile_u16<T>(W, ldw, kb, blockIdx.y * kMacro, blockIdx.x * kMacro, lds);
+    indep_tile_u16<T, F16>(W, ldw, kb, blockIdx.y * kMacro, blockIdx.x * kMacro, lds);
 }
 
 template <uint32_t T>
@@ -159,11 +173,21 @@ __global__ __launch_bounds__(256) void indep_u32_kernel(uint32_t* __restrict__ W
 
 template <typename E>
 __global__ __launch_bounds__(256) void output_kernel(const E* __restrict__ W, size_t ldw, uint32_t n,
-                                                     uint32_t* __restrict__ out, size_t ldo) {
+                                                     uint32_t* __restrict__ out, size_t ldo, uint32_t bias) {
     const uint32_t i = blockIdx.x;
     const E* src = W + (size_t)i * ldw;
     uint32_t* dst = out + (size_t)i * ldo;
-    for (uint32_t j = threadIdx.x; j < n; j += kThreads) dst[j] = (i == j) ? 0u : (uint32_t)src[j];
+    for (uint32_t j = threadIdx.x; j < n; j += kThreads) {
+        const uint32_t w = (uint32_t)src[j];
+        dst[j] = (i == j || w == 0) ? 0u : w - bias;
+    }
+}
+
+__global__ void init_range_kernel(uint32_t* p) {
+    if (threadIdx.x == 0) {
+        p[0] = 0u;
+        p[1] = 0xFFFFFFFFu;
+    }
 }
 
 // Zeroes a few words with a kernel (keeps the pipeline free of memset nodes).
@@ -206,7 +230,7 @@ struct EventTimer {
 
 enum { kKindInit = 0, kKindDiag = 1, kKindPanel = 2, kKindIndep = 3, kKindOut = 4 };
 
-template <typename E, uint32_t T>
+template <typename E, uint32_t T, bool F16 = false>
 int run_rounds(E* W, size_t ldw, uint32_t n, uint32_t N, uint32_t* pivot_buf, hipStream_t s, EventTimer& tm,
                uint32_t* rounds_out) {
     const uint32_t nb = N / kMacro;
@@ -224,7 +248,7 @@ int run_rounds(E* W, size_t ldw, uint32_t n, uint32_t N, uint32_t* pivot_buf, hi
         tm.mark(kKindPanel);
         tm.mark(kKindIndep);
         if constexpr (sizeof(E) == 2)
-            indep_u16_kernel<T><<<dim3(nb, nb), kThreads, 0, s>>>(W, ldw, kb);
+            indep_u16_kernel<T, F16><<<dim3(nb, nb), kThreads, 0, s>>>(W, ldw, kb);
         else
             indep_u32_kernel<T><<<dim3(nb, nb), kThreads, 0, s>>>(W, ldw, kb);
         tm.mark(kKindIndep);
@@ -263,28 +287,35 @@ int closure_device_impl(DeviceContext* ctx, const uint32_t* d, uint32_t n, size_
         (void)hipEventRecord(e0, s);
     }
 
-    // width decision: exact criterion max(p0) <= 65535
-    bool use16;
+    // width decision: exact criterion max(p0) <= 65535; the 16-bit path runs f16-ordered
+    // when the nonzero p0 values span at most 0x7C00 - 0x0400 (offset into positive normals)
+    bool use16, f16 = false;
+    uint32_t bias = 0;
     if (flags & SE_FLAG_FORCE_U32) {
         use16 = false;
     } else {
         uint32_t* dmax = static_cast<uint32_t*>(scratch(ctx, kSlotSmall, 256));
         if (!dmax) return SE_ERR_ALLOC;
-        uint32_t hmax = 0;
+        uint32_t range[2] = {0, 0};
         tm.mark(kKindInit);
-        zero_words_kernel<<<1, 64, 0, s>>>(dmax, 1);
+        init_range_kernel<<<1, 64, 0, s>>>(dmax);
         dim3 g((n + kInitTile - 1) / kInitTile, (n + kInitTile - 1) / kInitTile);
         p0_max_kernel<<<g, kThreads, 0, s>>>(d, n, ld, dmax);
         tm.mark(kKindInit);
         SE_CHECK_LAUNCH("p0 init launch failed");
-        if (!hip_ok(hipMemcpyAsync(&hmax, dmax, sizeof(uint32_t), hipMemcpyDeviceToHost, s),
+        if (!hip_ok(hipMemcpyAsync(range, dmax, sizeof(range), hipMemcpyDeviceToHost, s),
                     "Failed to copy data from device to host") ||
             !hip_ok(hipStreamSynchronize(s), "CUDA operations did not complete successfully"))
             return SE_ERR_COPY;
-        use16 = hmax <= 0xFFFFu;
+        use16 = range[0] <= 0xFFFFu;
         if ((flags & SE_FLAG_FORCE_U16) && !use16) {
             set_error("strongest-path values exceed 65535; the 16-bit closure is not exact here");
             return SE_ERR_RANGE;
+        }
+        if (use16 && !(flags & SE_FLAG_NO_F16) && range[1] != 0xFFFFFFFFu &&
+            range[0] - range[1] <= 0x7C00u - 0x0400u) {
+            f16 = true;
+            bias = 0x0400u - range[1];  // modulo 2^32: v + bias = v - min + 0x0400
         }
     }
 
@@ -295,9 +326,9 @@ int closure_device_impl(DeviceContext* ctx, const uint32_t* d, uint32_t n, size_
     dim3 gi(N / kInitTile, N / kInitTile);
     tm.mark(kKindInit);
     if (use16)
-        p0_write_kernel<uint16_t><<<gi, kThreads, 0, s>>>(d, n, ld, static_cast<uint16_t*>(W), N, ldw);
+        p0_write_kernel<uint16_t><<<gi, kThreads, 0, s>>>(d, n, ld, static_cast<uint16_t*>(W), N, ldw, bias);
     else
-        p0_write_kernel<uint32_t><<<gi, kThreads, 0, s>>>(d, n, ld, static_cast<uint32_t*>(W), N, ldw);
+        p0_write_kernel<uint32_t><<<gi, kThreads, 0, s>>>(d, n, ld, static_cast<uint32_t*>(W), N, ldw, 0u);
     tm.mark(kKindInit);
     SE_CHECK_LAUNCH("p0 init launch failed");
 
@@ -310,9 +341,13 @@ int closure_device_impl(DeviceContext* ctx, const uint32_t* d, uint32_t n, size_
     if (!pivot_buf) return SE_ERR_ALLOC;
     uint32_t rounds = 0;
     int rc;
-    if (use16)
-        rc = T == 64 ? run_rounds<uint16_t, 64>(static_cast<uint16_t*>(W), ldw, n, N, pivot_buf, s, tm, &rounds)
-                     : run_rounds<uint16_t, 32>(static_cast<uint16_t*>(W), ldw, n, N, pivot_buf, s, tm, &rounds);
+    uint16_t* W16 = static_cast<uint16_t*>(W);
+    if (use16 && f16)
+        rc = T == 64 ? run_rounds<uint16_t, 64, true>(W16, ldw, n, N, pivot_buf, s, tm, &rounds)
+                     : run_rounds<uint16_t, 32, true>(W16, ldw, n, N, pivot_buf, s, tm, &rounds);
+    else if (use16)
+        rc = T == 64 ? run_rounds<uint16_t, 64>(W16, ldw, n, N, pivot_buf, s, tm, &rounds)
+                     : run_rounds<uint16_t, 32>(W16, ldw, n, N, pivot_buf, s, tm, &rounds);
     else
         rc = T == 64 ? run_rounds<uint32_t, 64>(static_cast<uint32_t*>(W), ldw, n, N, pivot_buf, s, tm, &rounds)
                      : run_rounds<uint32_t, 32>(static_cast<uint32_t*>(W), ldw, n, N, pivot_buf, s, tm, &rounds);
@@ -321,9 +356,9 @@ int closure_device_impl(DeviceContext* ctx, const uint32_t* d, uint32_t n, size_
     dim3 go(n);
     tm.mark(kKindOut);
     if (use16)
-        output_kernel<uint16_t><<<go, kThreads, 0, s>>>(static_cast<uint16_t*>(W), ldw, n, out, ldo);
+        output_kernel<uint16_t><<<go, kThreads, 0, s>>>(static_cast<uint16_t*>(W), ldw, n, out, ldo, bias);
     else
-        output_kernel<uint32_t><<<go, kThreads, 0, s>>>(static_cast<uint32_t*>(W), ldw, n, out, ldo);
+        output_kernel<uint32_t><<<go, kThreads, 0, s>>>(static_cast<uint32_t*>(W), ldw, n, out, ldo, 0u);
     tm.mark(kKindOut);
     SE_CHECK_LAUNCH("output launch failed");
 
@@ -344,6 +379,7 @@ int closure_device_impl(DeviceContext* ctx, const uint32_t* d, uint32_t n, size_
         t.rounds = rounds;
         t.pivot = T;
         t.width_bits = use16 ? 16 : 32;
+        t.f16_ordered = f16 ? 1u : 0u;
         t.padded_n = N;
         t.indep_cells = (uint64_t)rounds * (uint64_t)(N - T) * (uint64_t)(N - T) * T;
         *timing = t;

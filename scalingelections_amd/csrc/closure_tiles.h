@@ -23,6 +23,9 @@ constexpr uint32_t kThreads = 256;
 __device__ __forceinline__ u16x2 as_u16x2(uint32_t x) { return __builtin_bit_cast(u16x2, x); }
 __device__ __forceinline__ uint32_t as_u32(u16x2 x) { return __builtin_bit_cast(uint32_t, x); }
 __device__ __forceinline__ u16x2 pk_min(u16x2 a, u16x2 b) { return __builtin_elementwise_min(a, b); }
+typedef _Float16 h16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ h16x2 as_h16x2(uint32_t x) { return __builtin_bit_cast(h16x2, x); }
+__device__ __forceinline__ uint32_t h_as_u32(h16x2 x) { return __builtin_bit_cast(uint32_t, x); }
 __device__ __forceinline__ u16x2 pk_max(u16x2 a, u16x2 b) { return __builtin_elementwise_max(a, b); }
 
 __device__ __forceinline__ uint32_t word(const uint4& v, int i) {
@@ -183,7 +186,11 @@ constexpr uint32_t indep_u16_lds_elems() { return kMacro * (T + 8) + T * kMacro;
 
 // C(I0.., J0..) = max(C, A (x) B) on one 128 x 128 macro tile; As/Bs are LDS buffers of
 // indep_u16_lds_elems<T>() u16 in total.  Starts with a barrier (LDS reuse between calls).
-template <uint32_t T>
+// F16 = true: the 16-bit values are bit patterns of non-negative f16 numbers (0 or within
+// [0x0400, 0x7C00] after an order-preserving offset), whose float order equals their
+// unsigned order; IEEE-754-2019 minimum/maximum return one operand unchanged, so two pivot
+// steps fold into 2 v_pk_minimum3_f16 + 1 v_pk_maximum3_f16 (0.75 VALU per update).
+template <uint32_t T, bool F16 = false>
 __device__ __forceinline__ void indep_tile_u16(uint16_t* __restrict__ W, size_t ldw, uint32_t kb, uint32_t I0,
                                                uint32_t J0, uint16_t* __restrict__ lds) {
     constexpr uint32_t AP = T + 8;  // A strip pitch (u16), keeps 16-byte alignment
@@ -221,16 +228,58 @@ __device__ __forceinline__ void indep_tile_u16(uint16_t* __restrict__ W, size_t 
 #pragma unroll
         for (int r = 0; r < 8; ++r)
             a4[r] = *reinterpret_cast<const uint4*>(As + (ty + 16 * r) * AP + k0);
+        if constexpr (F16) {
 #pragma unroll
-        for (int kk = 0; kk < 8; ++kk) {
-            const uint4 b4 = *reinterpret_cast<const uint4*>(Bs + (k0 + kk) * kMacro + tx * 8);
-            const u16x2 b[4] = {as_u16x2(b4.x), as_u16x2(b4.y), as_u16x2(b4.z), as_u16x2(b4.w)};
+            for (int kk = 0; kk < 8; kk += 2) {
+                const uint4 b4 = *reinterpret_cast<const uint4*>(Bs + (k0 + kk) * kMacro + tx * 8);
+                const uint4 b5 = *reinterpret_cast<const uint4*>(Bs + (k0 + kk + 1) * kMacro + tx * 8);
+                const h16x2 b0[4] = {as_h16x2(b4.x), as_h16x2(b4.y), as_h16x2(b4.z), as_h16x2(b4.w)};
+                const h16x2 b1[4] = {as_h16x2(b5.x), as_h16x2(b5.y), as_h16x2(b5.z), as_h16x2(b5.w)};
+                // all minimums of two rows first, then their maximums: every maximum3 reads
+                // results produced several instructions earlier (no dependency stalls)
 #pragma unroll
-            for (int r = 0; r < 8; ++r) {
-                const u16x2 a2 = as_u16x2(word(a4[r], kk >> 1));
-                const u16x2 av = (kk & 1) ? a2.yy : a2.xx;
+                for (int r = 0; r < 8; r += 2) {
+                    h16x2 m0[2][4], m1[2][4];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) acc[r][q] = pk_max(acc[r][q], pk_min(av, b[q]));
+                    for (int rr = 0; rr < 2; ++rr) {
+                        const h16x2 a2 = as_h16x2(word(a4[r + rr], kk >> 1));
+                        const h16x2 a0 = a2.xx, a1 = a2.yy;
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            m0[rr][q] = __builtin_elementwise_minimum(a0, b0[q]);
+                            m1[rr][q] = __builtin_elementwise_minimum(a1, b1[q]);
+                        }
+                    }
+#pragma unroll
+                    for (int rr = 0; rr < 2; ++rr)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const h16x2 cur = as_h16x2(as_u32(acc[r + rr][q]));
+                            acc[r + rr][q] = as_u16x2(h_as_u32(__builtin_elementwise_maximum(
+                                __builtin_elementwise_maximum(cur, m0[rr][q]), m1[rr][q])));
+                        }
+                }
+            }
+        } else {
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk) {
+                const uint4 b4 = *reinterpret_cast<const uint4*>(Bs + (k0 + kk) * kMacro + tx * 8);
+                const u16x2 b[4] = {as_u16x2(b4.x), as_u16x2(b4.y), as_u16x2(b4.z), as_u16x2(b4.w)};
+#pragma unroll
+                for (int r = 0; r < 8; r += 2) {
+                    u16x2 mn[2][4];
+#pragma unroll
+                    for (int rr = 0; rr < 2; ++rr) {
+                        const u16x2 a2 = as_u16x2(word(a4[r + rr], kk >> 1));
+                        const u16x2 av = (kk & 1) ? a2.yy : a2.xx;
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) mn[rr][q] = pk_min(av, b[q]);
+                    }
+#pragma unroll
+                    for (int rr = 0; rr < 2; ++rr)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) acc[r + rr][q] = pk_max(acc[r + rr][q], mn[rr][q]);
+                }
             }
         }
     }

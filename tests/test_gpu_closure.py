@@ -79,20 +79,52 @@ def test_width_selection_is_exact(gpu_lib, oracle_mod, hi, width):
             _timed(d, _lib.SE_FLAG_FORCE_U16)
 
 
-@pytest.mark.parametrize("width", ["u16", "u32"])
+@pytest.mark.parametrize("width", ["f16", "u16", "u32"])
 @pytest.mark.parametrize("pivot", [32, 64])
 @pytest.mark.parametrize("n", [1, 63, 64, 65, 200, 511, 640])
 def test_every_width_and_pivot_depth(gpu_lib, oracle_mod, width, pivot, n):
     from scalingelections_amd import _lib
 
     rng = np.random.default_rng(n * 7 + pivot)
-    d = rng.integers(0, 60000, (n, n)).astype(np.uint32)
-    flags = (_lib.SE_FLAG_FORCE_U16 if width == "u16" else _lib.SE_FLAG_FORCE_U32)
+    hi = 20000 if width == "f16" else 60000
+    d = rng.integers(0, hi, (n, n)).astype(np.uint32)
+    flags = {"f16": 0, "u16": _lib.SE_FLAG_FORCE_U16 | _lib.SE_FLAG_NO_F16, "u32": _lib.SE_FLAG_FORCE_U32}[width]
     flags |= _lib.SE_FLAG_PIVOT32 if pivot == 32 else _lib.SE_FLAG_PIVOT64
     got, t = _timed(d, flags)
-    assert (t["pivot"], t["width_bits"]) == (pivot, 16 if width == "u16" else 32)
+    assert (t["pivot"], t["width_bits"]) == (pivot, 32 if width == "u32" else 16)
+    assert t["f16_ordered"] == (1 if (width == "f16" and n > 1) else 0)
     assert t["rounds"] == (n + pivot - 1) // pivot
     assert np.array_equal(got, oracle_mod.closure_tiled(d, 16))
+
+
+def _ranged(n, lo, hi, seed):
+    """d whose nonzero p0 values are exactly the upper triangle, spanning [lo, hi]."""
+    rng = np.random.default_rng(seed)
+    d = np.triu(rng.integers(lo, hi + 1, (n, n), dtype=np.uint64), 1).astype(np.uint32)
+    d[0, 1], d[0, 2] = lo, hi
+    return d
+
+
+@pytest.mark.parametrize("lo,hi,f16", [
+    (1, 30721, True),            # span 30720: top value lands on 0x7C00 (+inf), still ordered
+    (1, 30722, False),           # span 30721: one past the f16-ordered range -> integer min/max
+    (34815, 65535, True),        # span 30720 at the top of the 16-bit range
+    (40000, 40100, True),
+    (7, 7, True),                # a single nonzero value
+])
+def test_f16_ordered_mode_boundaries(gpu_lib, oracle_mod, lo, hi, f16):
+    from scalingelections_amd import _lib
+
+    for n in (130, 300):
+        d = _ranged(n, lo, hi, lo + n)
+        want = oracle_mod.closure_tiled(d, 16)
+        got, t = _timed(d, 0)
+        assert t["width_bits"] == 16 and t["f16_ordered"] == (1 if f16 else 0)
+        assert np.array_equal(got, want)
+        got2, t2 = _timed(d, _lib.SE_FLAG_NO_F16)
+        assert t2["f16_ordered"] == 0 and np.array_equal(got2, want)
+        got3, t3 = _timed(d, _lib.SE_FLAG_PIVOT32)
+        assert t3["pivot"] == 32 and np.array_equal(got3, want)
 
 
 def test_degenerate_inputs(gpu_lib, oracle_mod):
