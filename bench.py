@@ -43,11 +43,15 @@ sys.path.insert(0, ROOT)
 
 import numpy as np  # noqa: E402
 
-# MI355X: 256 CUs x 64 lanes x 2.4 GHz = 3.93e13 32-bit VALU ops/s; packed 16-bit ops do
-# two per lane (7.86e13/s).  One cell update = 2 integer ops (min, max); one counted
-# pair-vote = 2 ops (compare, add with carry).
-PEAK_OPS_U32 = 256 * 64 * 2.4e9
-PEAK_OPS_U16 = 2 * PEAK_OPS_U32
+# MI355X VALU issue rate: 256 CUs x 64 lanes x 2.4 GHz = 3.93e13 lane-instructions/s.
+# Cell updates (c = max(c, min(a, b))) per lane-instruction of the sequence each path uses:
+#   f16-ordered 16-bit: 2 v_pk_minimum3_f16 + 1 v_pk_maximum3_f16 per 2 cells x 2 steps -> 4/3
+#   integer 16-bit:     1 v_pk_min_u16 + 1 v_pk_max_u16 per 2 cells                       -> 1
+#   32-bit:             2 v_min_u32 + 1 v_max3_u32 per cell x 2 steps                       -> 2/3
+# One counted pair-vote = 2 lane-instructions (compare, add with carry).
+VALU_LANE_INSTR = 256 * 64 * 2.4e9
+PEAK_OPS_U32 = VALU_LANE_INSTR
+UPDATES_PER_INSTR = {"f16": 4.0 / 3.0, "u16": 1.0, "u32": 2.0 / 3.0}
 PUBLISHED_TCELLS_32768 = 3.5  # reference README, Mojo on MI355X, end-to-end call
 
 
@@ -270,7 +274,19 @@ def main() -> int:
     ap.add_argument("--no-e2e", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the size sweep, counter, batch and serving")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device (runtime copy kernel) and exit: the known-size "
+                         "reference for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
     args = ap.parse_args()
+    if args.copy_calibration:
+        from scalingelections_amd import devmem
+
+        nbytes = args.copy_calibration << 20
+        a, b = devmem.DeviceBuffer(nbytes), devmem.DeviceBuffer(nbytes)
+        devmem._ok(devmem.hip().hipMemcpy(b.ptr, a.ptr, nbytes, devmem.hipMemcpyDeviceToDevice), "hipMemcpy D2D")
+        devmem.device_synchronize()
+        print(json.dumps({"copy_calibration_bytes": nbytes}), flush=True)
+        return 0
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: this build runs on one device; how the path would shard over "
@@ -307,13 +323,14 @@ def main() -> int:
     ms_per_step = elapsed * 1e3 / max(1, args.steps)
     value = float(n) ** 3 / (ms_per_step / 1e3) / 1e12
     width = timings[-1]["width_bits"] if timings else 0
-    launches = sum(t["rounds"] for t in timings)
+    mode = "u32" if width == 32 else ("f16" if timings and timings[-1]["f16_ordered"] else "u16")
+    rounds_run = sum(t["rounds"] for t in timings)
     indep_ms = sum(t["indep_ms"] for t in timings)
     indep_cells = sum(t["indep_cells"] for t in timings)
-    avg_launch_ms = indep_ms / max(1, launches)
-    ops_per_launch = 2.0 * indep_cells / max(1, launches)
-    achieved = ops_per_launch / (avg_launch_ms / 1e3) / 1e12 if avg_launch_ms > 0 else 0.0
-    peak = (PEAK_OPS_U16 if width == 16 else PEAK_OPS_U32) / 1e12
+    avg_round_ms = indep_ms / max(1, rounds_run)
+    cells_per_round = indep_cells / max(1, rounds_run)
+    achieved = cells_per_round / (avg_round_ms / 1e3) / 1e12 if avg_round_ms > 0 else 0.0
+    peak = VALU_LANE_INSTR * UPDATES_PER_INSTR.get(mode, 1.0) / 1e12
     del p_dev
 
     e2e = None
@@ -357,7 +374,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": round(value / PUBLISHED_TCELLS_32768, 3) if n == 32768 else None,
-        "dtype": f"u{width}",
+        "dtype": {"f16": "u16 (f16-ordered)", "u16": "u16", "u32": "u32"}.get(mode, f"u{width}"),
         "data": "synthetic",
         "config": {
             "workload": f"single election, {n} candidates, synthesized preferences (Binomial(65535, "
@@ -369,16 +386,20 @@ def main() -> int:
             "parallelism": "single GPU",
         },
         "roofline": {
-            "kernel": f"indep_u{width}_kernel (independent tiles)",
+            "kernel": f"indep_u{width}_kernel (independent tiles, {mode} instruction mix)",
             "bound": "valu",
             "achieved": round(achieved, 3),
             "peak": round(peak, 2),
-            "unit": "Tops/s (integer)",
+            "unit": "TCells/s (cell updates; 2 integer ops each)",
             "frac": round(achieved / peak, 4) if peak else None,
             "traffic": None,
-            "ops_per_launch": ops_per_launch,
-            "avg_launch_ms": round(avg_launch_ms, 4),
-            "launches": launches,
+            "peak_basis": "256 CUs x 64 lanes x 2.4 GHz VALU issue x cell updates per lane-instruction "
+                          "(f16-ordered 4/3, packed u16 1, u32 2/3)",
+            "cells_per_round": cells_per_round,
+            "avg_round_ms": round(avg_round_ms, 4),
+            "rounds_timed": rounds_run,
+            "note": "per round the independent tiles run as two launches (next pivot's strips, then the rest); "
+                    "avg_round_ms sums both, from HIP events on the library stream",
         },
         "phases_ms_per_step": {k: round(sum(t[k] for t in timings) / max(1, len(timings)), 2)
                                for k in ("total_ms", "init_ms", "panel_ms", "indep_ms", "output_ms")},

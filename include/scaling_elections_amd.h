@@ -54,6 +54,8 @@ extern "C" {
 #define SE_FLAG_PIVOT64 32u         /* for 32-bit); the result never depends on it         */
 #define SE_FLAG_NO_F16 128u         /* 16-bit path: keep integer min/max even when the     */
                                     /* values fit the f16-ordered range                   */
+#define SE_FLAG_NO_LOOKAHEAD 256u   /* one stream, no overlap of the next round's pivot    */
+                                    /* and panels with the current independent tiles      */
 
 /* Per-call measurements, filled when a non-NULL pointer is passed. */
 typedef struct se_timing {

@@ -31,6 +31,7 @@ SE_FLAG_ALLOW_TMA = 8
 SE_FLAG_PIVOT32 = 16
 SE_FLAG_PIVOT64 = 32
 SE_FLAG_NO_F16 = 128
+SE_FLAG_NO_LOOKAHEAD = 256
 
 # Every symbol include/scaling_elections_amd.h declares.
 EXPORTED = (

@@ -156,16 +156,44 @@ __global__ __launch_bounds__(256) void commit_pivot_kernel(E* __restrict__ W, si
 // ---------------------------------------------------------------------------------------
 // Independent tiles: one 128 x 128 macro tile per workgroup (indep_tile_u16 / _u32).
 
+// Tile selection: which = 0 -> every macro tile (grid nb x nb); which = 1 -> the tiles of
+// macro row/column `mnext`, i.e. the strips of the next pivot block (grid 2 nb - 1);
+// which = 2 -> every other tile (grid nb x nb, strip tiles exit at once).
+__device__ __forceinline__ bool select_tile(uint32_t which, uint32_t mnext, uint32_t nb, uint32_t* bi,
+                                            uint32_t* bj) {
+    if (which == 1) {
+        const uint32_t t = blockIdx.x;
+        if (t < nb) {
+            *bi = mnext;
+            *bj = t;
+        } else {
+            const uint32_t r = t - nb;
+            *bi = r < mnext ? r : r + 1;
+            *bj = mnext;
+        }
+        return true;
+    }
+    *bi = blockIdx.y;
+    *bj = blockIdx.x;
+    return !(which == 2 && (*bi == mnext || *bj == mnext));
+}
+
 template <uint32_t T, bool F16>
-__global__ __launch_bounds__(256) void indep_u16_kernel(uint16_t* __restrict__ W, size_t ldw, uint32_t kb) {
+__global__ __launch_bounds__(256) void indep_u16_kernel(uint16_t* __restrict__ W, size_t ldw, uint32_t kb,
+                                                        uint32_t which, uint32_t mnext, uint32_t nb) {
     __shared__ __align__(16) uint16_t lds[indep_u16_lds_elems<T>()];
-    indep_tile_u16<T, F16>(W, ldw, kb, blockIdx.y * kMacro, blockIdx.x * kMacro, lds);
+    uint32_t bi, bj;
+    if (!select_tile(which, mnext, nb, &bi, &bj)) return;
+    indep_tile_u16<T, F16>(W, ldw, kb, bi * kMacro, bj * kMacro, lds);
 }
 
 template <uint32_t T>
-__global__ __launch_bounds__(256) void indep_u32_kernel(uint32_t* __restrict__ W, size_t ldw, uint32_t kb) {
+__global__ __launch_bounds__(256) void indep_u32_kernel(uint32_t* __restrict__ W, size_t ldw, uint32_t kb,
+                                                        uint32_t which, uint32_t mnext, uint32_t nb) {
     __shared__ __align__(16) uint32_t lds[indep_u32_lds_elems<T>()];
-    indep_tile_u32<T>(W, ldw, kb, blockIdx.y * kMacro, blockIdx.x * kMacro, lds);
+    uint32_t bi, bj;
+    if (!select_tile(which, mnext, nb, &bi, &bj)) return;
+    indep_tile_u32<T>(W, ldw, kb, bi * kMacro, bj * kMacro, lds);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -205,7 +233,7 @@ struct EventTimer {
     hipStream_t s;
     std::vector<hipEvent_t> ev;
     std::vector<int> kind;
-    EventTimer(bool enabled, hipStream_t stream) : on(enabled), s(stream) {}
+    EventTimer(bool enabled, hipStream_t stream) : on(enabled && stream != nullptr), s(stream) {}
     ~EventTimer() {
         for (auto e : ev) (void)hipEventDestroy(e);
     }
@@ -230,37 +258,84 @@ struct EventTimer {
 
 enum { kKindInit = 0, kKindDiag = 1, kKindPanel = 2, kKindIndep = 3, kKindOut = 4 };
 
+bool ensure_side_stream(DeviceContext* ctx) {
+    if (ctx->side) return true;
+    if (!hip_ok(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking), "Failed to create a HIP stream") ||
+        !hip_ok(hipEventCreateWithFlags(&ctx->ev_strips, hipEventDisableTiming), "Failed to create events") ||
+        !hip_ok(hipEventCreateWithFlags(&ctx->ev_panel, hipEventDisableTiming), "Failed to create events"))
+        return false;
+    return true;
+}
+
+template <typename E, uint32_t T, bool F16>
+void launch_indep(E* W, size_t ldw, uint32_t kb, uint32_t which, uint32_t mnext, uint32_t nb, hipStream_t s) {
+    const dim3 grid = which == 1 ? dim3(2 * nb - 1) : dim3(nb, nb);
+    if constexpr (sizeof(E) == 2)
+        indep_u16_kernel<T, F16><<<grid, kThreads, 0, s>>>(W, ldw, kb, which, mnext, nb);
+    else
+        indep_u32_kernel<T><<<grid, kThreads, 0, s>>>(W, ldw, kb, which, mnext, nb);
+}
+
+// Rounds of the blocked closure.  With look-ahead (default, >= 2 rounds): round r's
+// independent kernel is split into the tiles of the next pivot's strips, launched first,
+// and the rest; the next round's pivot/panel kernel runs on the side stream as soon as
+// the strips are done, overlapping the rest of round r.  Neither side touches what the
+// other writes: the rest kernel excludes the strip tiles and does not read the pivot block.
 template <typename E, uint32_t T, bool F16 = false>
-int run_rounds(E* W, size_t ldw, uint32_t n, uint32_t N, uint32_t* pivot_buf, hipStream_t s, EventTimer& tm,
-               uint32_t* rounds_out) {
+int run_rounds(DeviceContext* ctx, E* W, size_t ldw, uint32_t n, uint32_t N, uint32_t* pivot_buf, hipStream_t s,
+               EventTimer& tm, EventTimer& tm_side, bool lookahead, uint32_t* rounds_out) {
     const uint32_t nb = N / kMacro;
     const uint32_t rounds = (n + T - 1) / T;
+    auto pivot_of = [&](uint32_t r) { return pivot_buf + (r & 1) * T * T; };
+    if (rounds == 0) {
+        *rounds_out = 0;
+        return SE_OK;
+    }
+    lookahead = lookahead && rounds >= 2;
+    if (lookahead && !ensure_side_stream(ctx)) return SE_ERR_DEVICE;
+    tm.mark(kKindPanel);
+    pivot_panel_kernel<E, T><<<2 * nb, kThreads, 0, s>>>(W, ldw, 0, nb, pivot_of(0), nullptr, 0);
+    tm.mark(kKindPanel);
     for (uint32_t r = 0; r < rounds; ++r) {
         if (g_interrupted.load(std::memory_order_relaxed)) {
             set_error("Stopped by signal");
             return SE_ERR_INTERRUPTED;
         }
         const uint32_t kb = r * T;
-        uint32_t* out = pivot_buf + (r & 1) * T * T;
-        const uint32_t* prev = r ? pivot_buf + ((r - 1) & 1) * T * T : nullptr;
-        tm.mark(kKindPanel);
-        pivot_panel_kernel<E, T><<<2 * nb, kThreads, 0, s>>>(W, ldw, kb, nb, out, prev, kb - T);
-        tm.mark(kKindPanel);
-        tm.mark(kKindIndep);
-        if constexpr (sizeof(E) == 2)
-            indep_u16_kernel<T, F16><<<dim3(nb, nb), kThreads, 0, s>>>(W, ldw, kb);
-        else
-            indep_u32_kernel<T><<<dim3(nb, nb), kThreads, 0, s>>>(W, ldw, kb);
-        tm.mark(kKindIndep);
+        const bool next = r + 1 < rounds;
+        if (!next) {
+            tm.mark(kKindIndep);
+            launch_indep<E, T, F16>(W, ldw, kb, 0, 0, nb, s);
+            tm.mark(kKindIndep);
+        } else if (!lookahead) {
+            tm.mark(kKindIndep);
+            launch_indep<E, T, F16>(W, ldw, kb, 0, 0, nb, s);
+            tm.mark(kKindIndep);
+            tm.mark(kKindPanel);
+            pivot_panel_kernel<E, T><<<2 * nb, kThreads, 0, s>>>(W, ldw, kb + T, nb, pivot_of(r + 1), pivot_of(r), kb);
+            tm.mark(kKindPanel);
+        } else {
+            const uint32_t mnext = (kb + T) / kMacro;
+            tm.mark(kKindIndep);
+            launch_indep<E, T, F16>(W, ldw, kb, 1, mnext, nb, s);
+            if (!hip_ok(hipEventRecord(ctx->ev_strips, s), "") ||
+                !hip_ok(hipStreamWaitEvent(ctx->side, ctx->ev_strips, 0), ""))
+                return SE_ERR_DEVICE;
+            tm_side.mark(kKindPanel);
+            pivot_panel_kernel<E, T>
+                <<<2 * nb, kThreads, 0, ctx->side>>>(W, ldw, kb + T, nb, pivot_of(r + 1), pivot_of(r), kb);
+            tm_side.mark(kKindPanel);
+            if (!hip_ok(hipEventRecord(ctx->ev_panel, ctx->side), "")) return SE_ERR_DEVICE;
+            launch_indep<E, T, F16>(W, ldw, kb, 2, mnext, nb, s);
+            tm.mark(kKindIndep);
+            if (!hip_ok(hipStreamWaitEvent(s, ctx->ev_panel, 0), "")) return SE_ERR_DEVICE;
+        }
         SE_CHECK_LAUNCH("closure round launch failed");
     }
-    if (rounds) {
-        tm.mark(kKindPanel);
-        commit_pivot_kernel<E, T><<<1, kThreads, 0, s>>>(W, ldw, (rounds - 1) * T,
-                                                          pivot_buf + ((rounds - 1) & 1) * T * T);
-        tm.mark(kKindPanel);
-        SE_CHECK_LAUNCH("closure round launch failed");
-    }
+    tm.mark(kKindPanel);
+    commit_pivot_kernel<E, T><<<1, kThreads, 0, s>>>(W, ldw, (rounds - 1) * T, pivot_of(rounds - 1));
+    tm.mark(kKindPanel);
+    SE_CHECK_LAUNCH("closure round launch failed");
     *rounds_out = rounds;
     return SE_OK;
 }
@@ -342,15 +417,19 @@ int closure_device_impl(DeviceContext* ctx, const uint32_t* d, uint32_t n, size_
     uint32_t rounds = 0;
     int rc;
     uint16_t* W16 = static_cast<uint16_t*>(W);
+    uint32_t* W32 = static_cast<uint32_t*>(W);
+    const bool la = !(flags & SE_FLAG_NO_LOOKAHEAD);
+    if (la && !ensure_side_stream(ctx)) return SE_ERR_DEVICE;
+    EventTimer tm_side((flags & SE_FLAG_TIME_KERNELS) != 0, ctx->side);
     if (use16 && f16)
-        rc = T == 64 ? run_rounds<uint16_t, 64, true>(W16, ldw, n, N, pivot_buf, s, tm, &rounds)
-                     : run_rounds<uint16_t, 32, true>(W16, ldw, n, N, pivot_buf, s, tm, &rounds);
+        rc = T == 64 ? run_rounds<uint16_t, 64, true>(ctx, W16, ldw, n, N, pivot_buf, s, tm, tm_side, la, &rounds)
+                     : run_rounds<uint16_t, 32, true>(ctx, W16, ldw, n, N, pivot_buf, s, tm, tm_side, la, &rounds);
     else if (use16)
-        rc = T == 64 ? run_rounds<uint16_t, 64>(W16, ldw, n, N, pivot_buf, s, tm, &rounds)
-                     : run_rounds<uint16_t, 32>(W16, ldw, n, N, pivot_buf, s, tm, &rounds);
+        rc = T == 64 ? run_rounds<uint16_t, 64>(ctx, W16, ldw, n, N, pivot_buf, s, tm, tm_side, la, &rounds)
+                     : run_rounds<uint16_t, 32>(ctx, W16, ldw, n, N, pivot_buf, s, tm, tm_side, la, &rounds);
     else
-        rc = T == 64 ? run_rounds<uint32_t, 64>(static_cast<uint32_t*>(W), ldw, n, N, pivot_buf, s, tm, &rounds)
-                     : run_rounds<uint32_t, 32>(static_cast<uint32_t*>(W), ldw, n, N, pivot_buf, s, tm, &rounds);
+        rc = T == 64 ? run_rounds<uint32_t, 64>(ctx, W32, ldw, n, N, pivot_buf, s, tm, tm_side, la, &rounds)
+                     : run_rounds<uint32_t, 32>(ctx, W32, ldw, n, N, pivot_buf, s, tm, tm_side, la, &rounds);
     if (rc != SE_OK) return rc;
 
     dim3 go(n);
@@ -371,6 +450,7 @@ int closure_device_impl(DeviceContext* ctx, const uint32_t* d, uint32_t n, size_
         t.total_ms = ms;
         double sums[5] = {0, 0, 0, 0, 0};
         tm.collect(sums);
+        tm_side.collect(sums);
         t.init_ms = sums[kKindInit];
         t.diag_ms = sums[kKindDiag];
         t.panel_ms = sums[kKindPanel];

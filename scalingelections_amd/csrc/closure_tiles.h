@@ -204,15 +204,20 @@ __device__ __forceinline__ void indep_tile_u16(uint16_t* __restrict__ W, size_t 
 #pragma unroll
     for (int r = 0; r < 8; ++r) c[r] = *reinterpret_cast<const uint4*>(cp + (size_t)(16 * r) * ldw);
 
+    // strips; cells of the pivot block (K, K) only feed pivot-row/column cells, which are
+    // not stored, so they are not read (zeros instead): the next round's panel kernel may be
+    // committing that block concurrently
     for (uint32_t q = t; q < kMacro * (T / 8); q += kThreads) {
         uint32_t row = q / (T / 8), ch = q % (T / 8);
-        *reinterpret_cast<uint4*>(As + row * AP + ch * 8) =
-            *reinterpret_cast<const uint4*>(W + (size_t)(I0 + row) * ldw + kb + ch * 8);
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (I0 + row - kb >= T) v = *reinterpret_cast<const uint4*>(W + (size_t)(I0 + row) * ldw + kb + ch * 8);
+        *reinterpret_cast<uint4*>(As + row * AP + ch * 8) = v;
     }
     for (uint32_t q = t; q < T * 16; q += kThreads) {
         uint32_t row = q >> 4, ch = q & 15;
-        *reinterpret_cast<uint4*>(Bs + row * kMacro + ch * 8) =
-            *reinterpret_cast<const uint4*>(W + (size_t)(kb + row) * ldw + J0 + ch * 8);
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (J0 + ch * 8 - kb >= T) v = *reinterpret_cast<const uint4*>(W + (size_t)(kb + row) * ldw + J0 + ch * 8);
+        *reinterpret_cast<uint4*>(Bs + row * kMacro + ch * 8) = v;
     }
     __syncthreads();
 
@@ -322,15 +327,17 @@ __device__ __forceinline__ void indep_tile_u32(uint32_t* __restrict__ W, size_t 
         c[r][0] = *reinterpret_cast<const uint4*>(cp + (size_t)(16 * r) * ldw);
         c[r][1] = *reinterpret_cast<const uint4*>(cp + (size_t)(16 * r) * ldw + 64);
     }
-    for (uint32_t q = t; q < kMacro * (T / 4); q += kThreads) {
+    for (uint32_t q = t; q < kMacro * (T / 4); q += kThreads) {  // strips, pivot block as zeros
         uint32_t row = q / (T / 4), ch = q % (T / 4);
-        *reinterpret_cast<uint4*>(As + row * AP + ch * 4) =
-            *reinterpret_cast<const uint4*>(W + (size_t)(I0 + row) * ldw + kb + ch * 4);
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (I0 + row - kb >= T) v = *reinterpret_cast<const uint4*>(W + (size_t)(I0 + row) * ldw + kb + ch * 4);
+        *reinterpret_cast<uint4*>(As + row * AP + ch * 4) = v;
     }
     for (uint32_t q = t; q < T * 32; q += kThreads) {
         uint32_t row = q >> 5, ch = q & 31;
-        *reinterpret_cast<uint4*>(Bs + row * kMacro + ch * 4) =
-            *reinterpret_cast<const uint4*>(W + (size_t)(kb + row) * ldw + J0 + ch * 4);
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (J0 + ch * 4 - kb >= T) v = *reinterpret_cast<const uint4*>(W + (size_t)(kb + row) * ldw + J0 + ch * 4);
+        *reinterpret_cast<uint4*>(Bs + row * kMacro + ch * 4) = v;
     }
     __syncthreads();
 

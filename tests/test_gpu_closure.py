@@ -79,10 +79,11 @@ def test_width_selection_is_exact(gpu_lib, oracle_mod, hi, width):
             _timed(d, _lib.SE_FLAG_FORCE_U16)
 
 
+@pytest.mark.parametrize("schedule", ["lookahead", "serial"])
 @pytest.mark.parametrize("width", ["f16", "u16", "u32"])
 @pytest.mark.parametrize("pivot", [32, 64])
 @pytest.mark.parametrize("n", [1, 63, 64, 65, 200, 511, 640])
-def test_every_width_and_pivot_depth(gpu_lib, oracle_mod, width, pivot, n):
+def test_every_width_and_pivot_depth(gpu_lib, oracle_mod, schedule, width, pivot, n):
     from scalingelections_amd import _lib
 
     rng = np.random.default_rng(n * 7 + pivot)
@@ -90,6 +91,8 @@ def test_every_width_and_pivot_depth(gpu_lib, oracle_mod, width, pivot, n):
     d = rng.integers(0, hi, (n, n)).astype(np.uint32)
     flags = {"f16": 0, "u16": _lib.SE_FLAG_FORCE_U16 | _lib.SE_FLAG_NO_F16, "u32": _lib.SE_FLAG_FORCE_U32}[width]
     flags |= _lib.SE_FLAG_PIVOT32 if pivot == 32 else _lib.SE_FLAG_PIVOT64
+    if schedule == "serial":
+        flags |= _lib.SE_FLAG_NO_LOOKAHEAD
     got, t = _timed(d, flags)
     assert (t["pivot"], t["width_bits"]) == (pivot, 32 if width == "u32" else 16)
     assert t["f16_ordered"] == (1 if (width == "f16" and n > 1) else 0)
