@@ -37,12 +37,16 @@
  * same rule in scaling_elections.cu:495-503 and :742-752.
  * p[i][j] = d[i][j] if d[i][j] > d[j][i] else 0; p[i][i] = 0. */
 void oracle_p0(const uint32_t *d, int64_t n, int64_t stride, uint32_t *p, int64_t p_stride) {
-    for (int64_t i = 0; i < n; i++)
-        for (int64_t j = 0; j < n; j++) {
-            uint32_t v = 0;
-            if (i != j && d[i * stride + j] > d[j * stride + i]) v = d[i * stride + j];
-            p[i * p_stride + j] = v;
-        }
+    /* 64 x 64 blocks so the transposed operand d[j][i] stays in cache */
+#pragma omp parallel for schedule(dynamic)
+    for (int64_t bi = 0; bi < n; bi += 64)
+        for (int64_t bj = 0; bj < n; bj += 64)
+            for (int64_t i = bi; i < bi + 64 && i < n; i++)
+                for (int64_t j = bj; j < bj + 64 && j < n; j++) {
+                    uint32_t v = 0;
+                    if (i != j && d[i * stride + j] > d[j * stride + i]) v = d[i * stride + j];
+                    p[i * p_stride + j] = v;
+                }
 }
 
 /* Serial strongest paths: scaling_elections.py:91-125.

@@ -11,6 +11,7 @@ package's own serial CPU closure.
 from __future__ import annotations
 
 import argparse
+import os
 import time
 
 import numpy as np
@@ -106,7 +107,6 @@ def main(argv=None) -> int:
     print(f"  CPU tile: {cpu_tile_size} × {cpu_tile_size}")
     if args.run_gpu:
         print(f"  GPU tile: {gpu_tile_size} × {gpu_tile_size}")
-    import os
     print(f"  CPU threads: {os.cpu_count()}")
     print(f"  Warmup: {args.warmup}, Repeat: {args.repeat}")
     print()

@@ -65,11 +65,13 @@ __global__ __launch_bounds__(256) void batch_count_kernel(const int32_t* __restr
     const int32_t* pos = pos_all + em.pos_off;
     uint32_t* d = d_all + em.d_off;
     const uint32_t C = em.C, m = em.m;
-    if (it.w == 1) {
+    if (it.w == 1) {  // full 128 x 128 tile: positions staged through LDS
         count_tile(pos, m, C, C, d, C, it.y, it.z, Pi, Pj);
         return;
     }
-    count_small(pos, m, C, C, d, C, &Pi[0][0]);
+    // small election or edge tile: voter-parallel sub-block
+    const uint32_t I0 = it.y * kCT, J0 = it.z * kCT;
+    count_block(pos, m, C, C, I0, J0, min(kCT, C - I0), min(kCT, C - J0), d, C, &Pi[0][0]);
 }
 
 // ------------------------------------------------------------------------- closure
@@ -252,17 +254,18 @@ int batch_device_impl(DeviceContext* ctx, uint32_t E, const uint32_t* Cs, const 
         for (uint32_t e = 0; e < E; ++e) {
             const uint32_t C = Cs[e];
             if (!C) continue;
-            if (C <= kCT) {
-                items.push_back(make_uint4(e, 0, 0, 0));
-                cost.push_back((double)ms[e] * C * C / 256.0 + 64.0);
-            } else {
-                const uint32_t nb = (C + kCT - 1) / kCT;
-                for (uint32_t I = 0; I < nb; ++I)
-                    for (uint32_t J = I; J < nb; ++J) {
-                        items.push_back(make_uint4(e, I, J, 1));
-                        cost.push_back((double)ms[e] * 64.0 + 64.0);
-                    }
-            }
+            // items: full 128 x 128 tiles (mode 1) and voter-parallel sub-blocks (mode 0) for
+            // small elections and the partial tiles at the edge; cost ~ voters per lane
+            const uint32_t nb = (C + kCT - 1) / kCT;
+            for (uint32_t I = 0; I < nb; ++I)
+                for (uint32_t J = I; J < nb; ++J) {
+                    const uint32_t ri = std::min(kCT, C - I * kCT), rj = std::min(kCT, C - J * kCT);
+                    const bool full = ri == kCT && rj == kCT;
+                    const uint32_t nbi = (ri + 7) / 8, nbj = (rj + 7) / 8;
+                    const uint32_t blocks = I == J ? nbi * (nbi + 1) / 2 : nbi * nbj;
+                    items.push_back(make_uint4(e, I, J, full ? 1 : 0));
+                    cost.push_back(full ? (double)ms[e] + 16.0 : (double)ms[e] / (double)(kCThreads / blocks) + 16.0);
+                }
         }
         std::vector<uint32_t> idx(items.size());
         std::iota(idx.begin(), idx.end(), 0u);

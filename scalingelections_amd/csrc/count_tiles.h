@@ -71,16 +71,35 @@ __device__ __forceinline__ void count_tile(const int32_t* __restrict__ pos, uint
     }
 }
 
-// Whole election with n <= 128 candidates, voter-parallel: the n x n counters are cut
-// into 8 x 8 blocks (at most 256) and the 256 lanes into G = 256 / blocks voter groups;
-// group g counts voters g, g + G, ...; partial blocks are summed through LDS (`red`, at
-// least 256 * 8 words) and written as the full matrix (both triangles, zero diagonal).
-__device__ __forceinline__ void count_small(const int32_t* __restrict__ pos, uint32_t m, uint32_t n, size_t ldp,
+// Voter-parallel counting of one sub-block of d: rows [I0, I0 + ri) x columns [J0, J0 + rj)
+// (ri, rj <= 128) of an election with n candidates.  The sub-block is cut into 8 x 8 blocks
+// and the 256 lanes into G = 256 / blocks voter groups; group g counts voters g, g + G, ...
+// and the partial blocks are summed through LDS (`red`, at least 256 * 8 words).  On a
+// diagonal sub-block (I0 == J0) only blocks bi <= bj are counted; a strict order per voter
+// gives the lower ones by complement, d[j][i] = m - d[i][j] (the diagonal counts itself: 0).
+// Off-diagonal sub-blocks write d[i][j] and, by complement, d[j][i].
+__device__ __forceinline__ void count_block(const int32_t* __restrict__ pos, uint32_t m, uint32_t n, size_t ldp,
+                                            uint32_t I0, uint32_t J0, uint32_t ri, uint32_t rj,
                                             uint32_t* __restrict__ d, size_t ldd, int32_t* red) {
     const uint32_t t = threadIdx.x;
-    const uint32_t nblk = (n + 7) / 8, blocks = nblk * nblk, G = kCThreads / blocks;
+    const bool diag = I0 == J0;
+    const uint32_t nbi = (ri + 7) / 8, nbj = (rj + 7) / 8;
+    const uint32_t blocks = diag ? nbi * (nbi + 1) / 2 : nbi * nbj;
+    const uint32_t G = kCThreads / blocks;
     const uint32_t b = t % blocks, g = t / blocks;
-    const uint32_t bi = b / nblk, bj = b % nblk;
+    uint32_t bi, bj;
+    if (diag) {  // b-th pair bi <= bj, row-major over the upper triangle of blocks
+        uint32_t r = b;
+        bi = 0;
+        while (r >= nbi - bi) {
+            r -= nbi - bi;
+            ++bi;
+        }
+        bj = bi + r;
+    } else {
+        bi = b / nbj;
+        bj = b % nbj;
+    }
     uint32_t acc[8][8];
 #pragma unroll
     for (int a = 0; a < 8; ++a)
@@ -93,8 +112,8 @@ __device__ __forceinline__ void count_small(const int32_t* __restrict__ pos, uin
 #pragma unroll
             for (int a = 0; a < 8; ++a) {
                 const uint32_t ci = 8 * bi + a, cj = 8 * bj + a;
-                pi[a] = ci < n ? row[ci] : 0;
-                pj[a] = cj < n ? row[cj] : 0;
+                pi[a] = ci < ri ? row[I0 + ci] : 0;
+                pj[a] = cj < rj ? row[J0 + cj] : 0;
             }
 #pragma unroll
             for (int a = 0; a < 8; ++a)
@@ -112,10 +131,37 @@ __device__ __forceinline__ void count_small(const int32_t* __restrict__ pos, uin
             const uint32_t bb = q / 8, c = q % 8;
             uint32_t sum = 0;
             for (uint32_t gg = 0; gg < G; ++gg) sum += (uint32_t)red[(gg * blocks + bb) * 8 + c];
-            const uint32_t i = 8 * (bb / nblk) + a, j = 8 * (bb % nblk) + c;
-            if (i < n && j < n) d[(size_t)i * ldd + j] = sum;
+            uint32_t qi, qj;  // block coordinates of bb
+            if (diag) {
+                uint32_t r = bb;
+                qi = 0;
+                while (r >= nbi - qi) {
+                    r -= nbi - qi;
+                    ++qi;
+                }
+                qj = qi + r;
+            } else {
+                qi = bb / nbj;
+                qj = bb % nbj;
+            }
+            const uint32_t li = 8 * qi + a, lj = 8 * qj + c;
+            if (li >= ri || lj >= rj) continue;
+            const uint32_t i = I0 + li, j = J0 + lj;
+            if (diag && qi == qj) {
+                d[(size_t)i * ldd + j] = sum;  // diagonal block: both triangles counted
+            } else {
+                d[(size_t)i * ldd + j] = sum;
+                d[(size_t)j * ldd + i] = m - sum;
+            }
         }
     }
+    (void)n;
+}
+
+// Whole election with n <= 128 candidates (one diagonal sub-block).
+__device__ __forceinline__ void count_small(const int32_t* __restrict__ pos, uint32_t m, uint32_t n, size_t ldp,
+                                            uint32_t* __restrict__ d, size_t ldd, int32_t* red) {
+    count_block(pos, m, n, ldp, 0, 0, n, n, d, ldd, red);
 }
 
 }  // namespace se

@@ -262,3 +262,41 @@ def test_mallows_generator_matches_closed_form(phi):
     inv = sum((rank_of_ref[:, i] > rank_of_ref[:, j]).astype(np.int64) for i in range(C) for j in range(i + 1, C))
     mean_off = sum(sum(t * phi ** t for t in range(k + 1)) / Z[k] for k in range(C))
     assert abs(inv.mean() - mean_off) < 0.05
+
+
+def _no_device():
+    from scalingelections_amd import api
+
+    return api.device_count() == 0
+
+
+def test_gpu_paths_fail_loudly_without_a_device():
+    """No silent CPU fallback: every GPU entry point raises when no device is visible."""
+    from scalingelections_amd import build_pairwise_preferences, compute_strongest_paths, get_winner_and_ranking
+
+    if not _no_device():
+        pytest.skip("a HIP device is visible")
+    d = np.zeros((8, 8), np.uint32)
+    with pytest.raises(RuntimeError, match="No HIP devices available"):
+        compute_strongest_paths(d, allow_gpu=True, tile_size=32)
+    with pytest.raises(RuntimeError, match="No HIP devices available"):
+        build_pairwise_preferences([np.array([0, 1, 2])])
+    with pytest.raises(RuntimeError, match="No HIP devices available"):
+        get_winner_and_ranking([0, 1], np.zeros((2, 2), np.uint32))
+
+
+def test_ballot_validation_before_device_use():
+    from scalingelections_amd import _lib
+
+    ids = np.array([0, 5], np.int32)
+    off = np.array([0, 2], np.int64)
+    out = np.zeros((3, 3), np.uint32)
+    rc = _lib.lib().se_count_rankings(ids.ctypes.data, off.ctypes.data, 1, 3, out.ctypes.data, 3)
+    assert rc == _lib.SE_ERR_INVALID and "[0, n)" in _lib.last_error()
+    bad_off = np.array([2, 0], np.int64)
+    rc = _lib.lib().se_count_rankings(ids.ctypes.data, bad_off.ctypes.data, 1, 6, out.ctypes.data, 3)
+    assert rc == _lib.SE_ERR_INVALID
+    with pytest.raises(ValueError):
+        from scalingelections_amd import builder
+
+        builder.rankings_to_positions([np.array([0, -1])], 2)
