@@ -199,10 +199,16 @@ __device__ __forceinline__ void indep_tile_u16(uint16_t* __restrict__ W, size_t 
     const uint32_t t = threadIdx.x, tx = t & 15, ty = t >> 4;
     __syncthreads();
 
+    // C tile; pivot rows/columns are neither stored nor read here (their cells may be
+    // written concurrently by the next round's panel kernel)
     uint4 c[8];
     const uint16_t* cp = W + (size_t)(I0 + ty) * ldw + J0 + tx * 8;
+    const bool col_in_pivot = J0 + tx * 8 - kb < T;
 #pragma unroll
-    for (int r = 0; r < 8; ++r) c[r] = *reinterpret_cast<const uint4*>(cp + (size_t)(16 * r) * ldw);
+    for (int r = 0; r < 8; ++r) {
+        c[r] = make_uint4(0, 0, 0, 0);
+        if (!col_in_pivot && I0 + ty + 16 * r - kb >= T) c[r] = *reinterpret_cast<const uint4*>(cp + (size_t)(16 * r) * ldw);
+    }
 
     // strips; cells of the pivot block (K, K) only feed pivot-row/column cells, which are
     // not stored, so they are not read (zeros instead): the next round's panel kernel may be
@@ -320,12 +326,16 @@ __device__ __forceinline__ void indep_tile_u32(uint32_t* __restrict__ W, size_t 
     const uint32_t t = threadIdx.x, tx = t & 15, ty = t >> 4;
     __syncthreads();
 
-    uint4 c[8][2];
+    uint4 c[8][2];  // C tile without its pivot rows/columns (see indep_tile_u16)
     const uint32_t* cp = W + (size_t)(I0 + ty) * ldw + J0 + tx * 4;
+    const bool lo_in_pivot = J0 + tx * 4 - kb < T, hi_in_pivot = J0 + 64 + tx * 4 - kb < T;
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
-        c[r][0] = *reinterpret_cast<const uint4*>(cp + (size_t)(16 * r) * ldw);
-        c[r][1] = *reinterpret_cast<const uint4*>(cp + (size_t)(16 * r) * ldw + 64);
+        const bool row_ok = I0 + ty + 16 * r - kb >= T;
+        c[r][0] = make_uint4(0, 0, 0, 0);
+        c[r][1] = make_uint4(0, 0, 0, 0);
+        if (row_ok && !lo_in_pivot) c[r][0] = *reinterpret_cast<const uint4*>(cp + (size_t)(16 * r) * ldw);
+        if (row_ok && !hi_in_pivot) c[r][1] = *reinterpret_cast<const uint4*>(cp + (size_t)(16 * r) * ldw + 64);
     }
     for (uint32_t q = t; q < kMacro * (T / 4); q += kThreads) {  // strips, pivot block as zeros
         uint32_t row = q / (T / 4), ch = q % (T / 4);
