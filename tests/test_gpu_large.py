@@ -61,6 +61,11 @@ def test_c3_32768_closure(gpu_lib, oracle_mod, flags):
     assert t["width_bits"] == (32 if flags == "u32" else 16)
     assert _sha(p) == fx["p_sha256"]
     if flags == "auto":
+        from scalingelections_amd import wins_and_order
+
+        wins, order = wins_and_order(p)
+        assert _sha(wins.astype(np.int64), "<i8") == fx["wins_sha256"]
+        assert _sha(order.astype(np.int64), "<i8") == fx["ranking_sha256"]
         p0 = oracle_mod.p0(d)
         del d
         rng = np.random.default_rng(0)
