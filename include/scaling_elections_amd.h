@@ -56,6 +56,8 @@ extern "C" {
                                     /* values fit the f16-ordered range                   */
 #define SE_FLAG_NO_LOOKAHEAD 256u   /* one stream, no overlap of the next round's pivot    */
                                     /* and panels with the current independent tiles      */
+#define SE_FLAG_LOOKAHEAD 512u      /* overlap also below 8192 padded candidates (default  */
+                                    /* only from 8192 on)                                 */
 
 /* Per-call measurements, filled when a non-NULL pointer is passed. */
 typedef struct se_timing {

@@ -418,7 +418,9 @@ int closure_device_impl(DeviceContext* ctx, const uint32_t* d, uint32_t n, size_
     int rc;
     uint16_t* W16 = static_cast<uint16_t*>(W);
     uint32_t* W32 = static_cast<uint32_t*>(W);
-    const bool la = !(flags & SE_FLAG_NO_LOOKAHEAD);
+    // look-ahead pays once a round's independent work clearly outlasts the pivot/panel
+    // kernel (measured: slower at N = 4096, faster from 8192 on); SE_FLAG_LOOKAHEAD forces it
+    const bool la = !(flags & SE_FLAG_NO_LOOKAHEAD) && (N >= 8192 || (flags & SE_FLAG_LOOKAHEAD));
     if (la && !ensure_side_stream(ctx)) return SE_ERR_DEVICE;
     EventTimer tm_side((flags & SE_FLAG_TIME_KERNELS) != 0, ctx->side);
     if (use16 && f16)

@@ -91,8 +91,7 @@ def test_every_width_and_pivot_depth(gpu_lib, oracle_mod, schedule, width, pivot
     d = rng.integers(0, hi, (n, n)).astype(np.uint32)
     flags = {"f16": 0, "u16": _lib.SE_FLAG_FORCE_U16 | _lib.SE_FLAG_NO_F16, "u32": _lib.SE_FLAG_FORCE_U32}[width]
     flags |= _lib.SE_FLAG_PIVOT32 if pivot == 32 else _lib.SE_FLAG_PIVOT64
-    if schedule == "serial":
-        flags |= _lib.SE_FLAG_NO_LOOKAHEAD
+    flags |= _lib.SE_FLAG_NO_LOOKAHEAD if schedule == "serial" else _lib.SE_FLAG_LOOKAHEAD
     got, t = _timed(d, flags)
     assert (t["pivot"], t["width_bits"]) == (pivot, 32 if width == "u32" else 16)
     assert t["f16_ordered"] == (1 if (width == "f16" and n > 1) else 0)
