@@ -270,21 +270,27 @@ class Server:
         return ServeResult(reqs, wall, batches, sizes)
 
 
-def solo_result(pool: Workload, index: int, stream: Optional[devmem.Stream] = None):
+def solo_result(pool: Workload, index: int, stream: Optional[devmem.Stream] = None, timed: bool = False):
     """The election processed alone through the single-election entry points:
-    (ranking, sha256 of the strongest-path matrix)."""
+    (ranking, sha256 of the strongest-path matrix[, seconds from resident positions to
+    ranking on the host])."""
     L = _lib.lib()
     C, m = int(pool.C[index]), int(pool.m[index])
     pos = devmem.DeviceBuffer.from_array(pool.election_positions(index))
     d = devmem.DeviceBuffer(C * C * 4)
     p = devmem.DeviceBuffer(C * C * 4)
     sh = ctypes.c_void_p(stream.handle) if stream else None
+    devmem.device_synchronize()
+    t0 = time.perf_counter()
     _lib.check(L.se_count_device(pos.ptr, m, C, d.ptr, C, sh, None))
     _lib.check(L.se_closure_device(d.ptr, C, C, p.ptr, C, 0, sh, None))
     wins = np.empty(C, np.uint32)
     ranking = np.empty(C, np.uint32)
     _lib.check(L.se_rank_device(p.ptr, C, C, wins.ctypes.data, ranking.ctypes.data, sh))
+    elapsed = time.perf_counter() - t0
     out = np.empty((C, C), np.uint32)
     devmem.device_synchronize()
     p.download(out)
+    if timed:
+        return ranking, _sha(out), elapsed
     return ranking, _sha(out)
