@@ -113,9 +113,11 @@ int se_closure_cpu_serial(const uint32_t *prefs, uint32_t n, size_t row_stride, 
 
 /* ---- pairwise preference counter ---------------------------------------------------- */
 /* positions: m x n int32 (row v = voter v), positions[v][c] = place of candidate c in
- * voter v's ballot.  Every row must hold n distinct values (a strict order), which the
+ * voter v's ballot.  Every row must be a permutation of 0..n-1 (a strict order), which the
  * builder guarantees by filling incomplete ballots.  prefs: n x n uint32 (row stride
- * prefs_stride) receives d[i][j] = #{v : positions[v][i] < positions[v][j]}. */
+ * prefs_stride) receives d[i][j] = #{v : positions[v][i] < positions[v][j]}.
+ * se_count returns SE_ERR_INVALID when a position lies outside the range the counter packs
+ * (below 128 for n <= 128, below 32768 for n <= 32768); se_count_device does not check. */
 int se_count(const int32_t *positions, uint32_t m, uint32_t n, uint32_t *prefs,
              size_t prefs_stride);
 int se_count_device(const int32_t *positions_dev, uint32_t m, uint32_t n, uint32_t *prefs_dev,

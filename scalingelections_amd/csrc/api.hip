@@ -20,7 +20,8 @@ namespace se {
 int closure_device_impl(DeviceContext* ctx, const uint32_t* d, uint32_t n, size_t ld, uint32_t* out,
                         size_t ldo, uint32_t flags, hipStream_t s, se_timing* timing);
 int count_device_impl(DeviceContext* ctx, const int32_t* pos, uint32_t m, uint32_t n, uint32_t* d,
-                      size_t ldd, hipStream_t s, double* kernel_ms);
+                      size_t ldd, hipStream_t s, double* kernel_ms, uint32_t* flag);
+int clear_flag(uint32_t* flag, hipStream_t s);
 int count_rankings_device_impl(DeviceContext* ctx, const int32_t* ids, const int64_t* offsets,
                                uint32_t m, uint32_t n, uint32_t* d, size_t ldd, hipStream_t s);
 int wins_device_impl(DeviceContext* ctx, const uint32_t* p, uint32_t n, size_t ld, uint32_t* wins,
@@ -296,16 +297,26 @@ int se_count(const int32_t* positions, uint32_t m, uint32_t n, uint32_t* prefs, 
     if (!pos) return SE_ERR_ALLOC;
     uint32_t* d = static_cast<uint32_t*>(scratch(ctx, kSlotPrefs, dbytes));
     if (!d) return SE_ERR_ALLOC;
+    uint32_t* flag = static_cast<uint32_t*>(scratch(ctx, kSlotCountFlag, 256));
+    if (!flag) return SE_ERR_ALLOC;
     if (m > 0 && !hip_ok(hipMemcpyAsync(pos, positions, pbytes, hipMemcpyHostToDevice, s),
                          "Failed to copy data from host to device"))
         return SE_ERR_COPY;
-    int rc = count_device_impl(ctx, pos, m, n, d, n, s, nullptr);
+    int rc = clear_flag(flag, s);
+    if (rc == SE_OK) rc = count_device_impl(ctx, pos, m, n, d, n, s, nullptr, flag);
     if (rc != SE_OK) return rc;
+    uint32_t bad = 0;
     if (!hip_ok(hipMemcpy2DAsync(prefs, prefs_stride * sizeof(uint32_t), d, n * sizeof(uint32_t),
                                  n * sizeof(uint32_t), n, hipMemcpyDeviceToHost, s),
+                "Failed to copy data from device to host") ||
+        !hip_ok(hipMemcpyAsync(&bad, flag, sizeof(bad), hipMemcpyDeviceToHost, s),
                 "Failed to copy data from device to host"))
         return SE_ERR_COPY;
     if (!hip_ok(hipStreamSynchronize(s), "CUDA transfers did not complete successfully")) return SE_ERR_COPY;
+    if (bad) {
+        set_error("positions out of range: every ballot row must be a permutation of 0..n-1");
+        return SE_ERR_INVALID;
+    }
     return SE_OK;
 }
 
@@ -320,7 +331,7 @@ int se_count_device(const int32_t* positions_dev, uint32_t m, uint32_t n, uint32
     DeviceContext* ctx = context_for(stream);
     if (!ctx) return SE_ERR_NO_DEVICE;
     std::lock_guard<std::mutex> g(ctx->lock);
-    return count_device_impl(ctx, positions_dev, m, n, prefs_dev, prefs_stride, ctx->stream, kernel_ms);
+    return count_device_impl(ctx, positions_dev, m, n, prefs_dev, prefs_stride, ctx->stream, kernel_ms, nullptr);
 }
 
 int se_count_rankings(const int32_t* ids, const int64_t* offsets, uint32_t m, uint32_t n, uint32_t* prefs,

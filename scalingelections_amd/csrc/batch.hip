@@ -34,7 +34,7 @@
 namespace se {
 
 int count_device_impl(DeviceContext* ctx, const int32_t* pos, uint32_t m, uint32_t n, uint32_t* d, size_t ldd,
-                      hipStream_t s, double* kernel_ms);
+                      hipStream_t s, double* kernel_ms, uint32_t* flag);
 int closure_device_impl(DeviceContext* ctx, const uint32_t* d, uint32_t n, size_t ld, uint32_t* out,
                         size_t ldo, uint32_t flags, hipStream_t s, se_timing* timing);
 int wins_device_impl(DeviceContext* ctx, const uint32_t* p, uint32_t n, size_t ld, uint32_t* wins,
@@ -49,29 +49,39 @@ struct ElectionMeta {
     uint64_t d_off;    // element offset of d and of p (C x C uint32)
     uint64_t w_off;    // element offset of the work matrix (Cp x Cp)
     uint64_t r_off;    // element offset of the ranking (C uint32)
-    uint32_t C, m, Cp, pad;
+    uint32_t C, m, Cp, K;  // K: counter packing width (voters per word)
 };
 
 // ------------------------------------------------------------------------- counting
 // item.x = election, item.y = I, item.z = J, item.w = mode (0: small voter-parallel, 1: tile)
-__global__ __launch_bounds__(256) void batch_count_kernel(const int32_t* __restrict__ pos_all,
-                                                          uint32_t* __restrict__ d_all,
-                                                          const ElectionMeta* __restrict__ meta,
-                                                          const uint4* __restrict__ items) {
-    __shared__ __align__(16) int32_t Pi[kVC][kCT];
-    __shared__ __align__(16) int32_t Pj[kVC][kCT];
-    const uint4 it = items[blockIdx.x];
-    const ElectionMeta em = meta[it.x];
-    const int32_t* pos = pos_all + em.pos_off;
-    uint32_t* d = d_all + em.d_off;
-    const uint32_t C = em.C, m = em.m;
-    if (it.w == 1) {  // full 128 x 128 tile: positions staged through LDS
-        count_tile(pos, m, C, C, d, C, it.y, it.z, Pi, Pj);
+template <uint32_t K>
+__device__ __forceinline__ void batch_count_item(const int32_t* pos, uint32_t* d, uint32_t C, uint32_t m,
+                                                 const uint4 it, uint32_t (*PI)[kCT], uint32_t (*PJ)[kCT]) {
+    if (it.w == 1) {  // full 128 x 128 tile
+        count_tile<K>(pos, 0, m, C, C, d, C, it.y, it.z, PI, PJ, &PI[0][0], false);
         return;
     }
     // small election or edge tile: voter-parallel sub-block
     const uint32_t I0 = it.y * kCT, J0 = it.z * kCT;
-    count_block(pos, m, C, C, I0, J0, min(kCT, C - I0), min(kCT, C - J0), d, C, &Pi[0][0]);
+    count_block<K>(pos, 0, m, C, I0, J0, min(kCT, C - I0), min(kCT, C - J0), d, C, PI, PJ, &PI[0][0], false);
+}
+
+__global__ __launch_bounds__(256) void batch_count_kernel(const int32_t* __restrict__ pos_all,
+                                                          uint32_t* __restrict__ d_all,
+                                                          const ElectionMeta* __restrict__ meta,
+                                                          const uint4* __restrict__ items) {
+    __shared__ __align__(16) uint32_t PI[kVW][kCT];
+    __shared__ __align__(16) uint32_t PJ[kVW][kCT];
+    const uint4 it = items[blockIdx.x];
+    const ElectionMeta em = meta[it.x];
+    const int32_t* pos = pos_all + em.pos_off;
+    uint32_t* d = d_all + em.d_off;
+    if (em.K == 4)
+        batch_count_item<4>(pos, d, em.C, em.m, it, PI, PJ);
+    else if (em.K == 2)
+        batch_count_item<2>(pos, d, em.C, em.m, it, PI, PJ);
+    else
+        batch_count_item<1>(pos, d, em.C, em.m, it, PI, PJ);
 }
 
 // ------------------------------------------------------------------------- closure
@@ -206,7 +216,7 @@ int batch_device_impl(DeviceContext* ctx, uint32_t E, const uint32_t* Cs, const 
         for (uint32_t e = 0; e < E; ++e) {
             const uint32_t C = Cs[e], m = ms[e];
             if (C) {
-                int rc = count_device_impl(ctx, pos + po, m, C, dbuf, C, s, nullptr);
+                int rc = count_device_impl(ctx, pos + po, m, C, dbuf, C, s, nullptr, nullptr);
                 if (rc == SE_OK) rc = closure_device_impl(ctx, dbuf, C, C, paths + doff, C, 0, s, nullptr);
                 if (rc == SE_OK) rc = wins_device_impl(ctx, paths + doff, C, C, wins, s);
                 if (rc != SE_OK) return rc;
@@ -238,7 +248,7 @@ int batch_device_impl(DeviceContext* ctx, uint32_t E, const uint32_t* Cs, const 
             em.C = Cs[e];
             em.m = ms[e];
             em.Cp = em.C ? padded_side(em.C, T) : 0;
-            em.pad = 0;
+            em.K = count_pack(em.C, em.m);
             em.pos_off = po;
             em.d_off = doff;
             em.w_off = wo;
@@ -255,7 +265,7 @@ int batch_device_impl(DeviceContext* ctx, uint32_t E, const uint32_t* Cs, const 
             const uint32_t C = Cs[e];
             if (!C) continue;
             // items: full 128 x 128 tiles (mode 1) and voter-parallel sub-blocks (mode 0) for
-            // small elections and the partial tiles at the edge; cost ~ voters per lane
+            // small elections and the partial tiles at the edge
             const uint32_t nb = (C + kCT - 1) / kCT;
             for (uint32_t I = 0; I < nb; ++I)
                 for (uint32_t J = I; J < nb; ++J) {
@@ -264,7 +274,12 @@ int batch_device_impl(DeviceContext* ctx, uint32_t E, const uint32_t* Cs, const 
                     const uint32_t nbi = (ri + 7) / 8, nbj = (rj + 7) / 8;
                     const uint32_t blocks = I == J ? nbi * (nbi + 1) / 2 : nbi * nbj;
                     items.push_back(make_uint4(e, I, J, full ? 1 : 0));
-                    cost.push_back(full ? (double)ms[e] + 16.0 : (double)ms[e] / (double)(kCThreads / blocks) + 16.0);
+                    // per-lane instructions: staging (K loads + packing per word) and 128
+                    // per word shared by the G word groups of a voter-parallel item
+                    const uint32_t K = meta[e].K;
+                    const double words = (double)((ms[e] + K - 1) / K);
+                    const double G = full ? 1.0 : (double)(kCThreads / blocks);
+                    cost.push_back(words * (K + 4) + words * 128.0 / G + 64.0);
                 }
         }
         std::vector<uint32_t> idx(items.size());

@@ -7,7 +7,9 @@
 //     d[i][j] = #{v : positions[v][i] < positions[v][j]},
 // a dense compare-and-count over 128 x 128 tiles of d with no atomics.  Only tiles with
 // I <= J are computed: for a strict order per voter d[j][i] = m - d[i][j] (i != j).
-// Per voter a lane holds 8 + 8 positions and does 64 compare+add pairs (v_cmp + v_addc).
+// Positions are packed 4 or 2 voters to a word in LDS and compared with one subtract and
+// one signed dot product per word (count_tiles.h); above 32768 candidates, one voter per
+// word with compare-and-add.
 //
 // General ballots (duplicates allowed, exactly the reference's scatter) go through
 // count_rankings_kernel: one workgroup per ballot, atomic increments.
@@ -15,6 +17,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cstdlib>
 
 #include "count_tiles.h"
 #include "se_internal.h"
@@ -25,38 +28,41 @@ namespace {
 
 // One workgroup per (tile pair, voter slice).  Tile pairs (I, J), I <= J, are numbered
 // row-major over the upper triangle of 128 x 128 tiles (n <= 128: one voter-parallel item).
-// With several slices each writes partial counts of its voters to its own n x n plane of
-// `out` (the complement d[j][i] = m_slice - d[i][j] holds per slice); reduce_slices_kernel
-// sums the planes.
-__global__ __launch_bounds__(256) void count_kernel(const int32_t* __restrict__ pos, uint32_t m, uint32_t n,
-                                                    size_t ldp, uint32_t* __restrict__ out, size_t ldo,
-                                                    uint32_t nb, uint32_t slices, uint32_t slice_len,
-                                                    size_t plane) {
-    __shared__ __align__(16) int32_t Pi[kVC][kCT];
-    __shared__ __align__(16) int32_t Pj[kVC][kCT];
+// Slice s covers voters [v0 + s slice_len, v0 + (s + 1) slice_len) of [v0, v1).  With several
+// slices each writes partial counts of its voters to its own n x n plane of `out` (the
+// complement d[j][i] = m_slice - d[i][j] holds per slice); reduce_slices_kernel sums the
+// planes.  Out-of-range positions for the packing width set *flag.
+template <uint32_t K>
+__global__ __launch_bounds__(256) void count_kernel(const int32_t* __restrict__ pos, uint32_t v0, uint32_t v1,
+                                                    uint32_t n, size_t ldp, uint32_t* __restrict__ out, size_t ldo,
+                                                    uint32_t nb, uint32_t slices, uint32_t slice_len, size_t plane,
+                                                    uint32_t accumulate, uint32_t* __restrict__ flag) {
+    __shared__ __align__(16) uint32_t PI[kVW][kCT];
+    __shared__ __align__(16) uint32_t PJ[kVW][kCT];
     const uint32_t slice = blockIdx.x % slices;
     uint32_t t = blockIdx.x / slices;
-    const uint32_t vb = min(m, slice * slice_len), ve = min(m, vb + slice_len);
-    const int32_t* p = pos + (size_t)vb * ldp;
+    const uint32_t vb = min(v1, v0 + slice * slice_len), ve = min(v1, vb + slice_len);
     uint32_t* o = out + (size_t)slice * plane;
+    uint32_t bits;
     if (n <= kCT) {
-        count_small(p, ve - vb, n, ldp, o, ldo, &Pi[0][0]);
-        return;
+        bits = count_block<K>(pos, vb, ve, ldp, 0, 0, n, n, o, ldo, PI, PJ, &PI[0][0], accumulate != 0);
+    } else {
+        uint32_t I = 0;
+        while (t >= nb - I) {
+            t -= nb - I;
+            ++I;
+        }
+        bits = count_tile<K>(pos, vb, ve, n, ldp, o, ldo, I, I + t, PI, PJ, &PI[0][0], accumulate != 0);
     }
-    uint32_t I = 0;
-    while (t >= nb - I) {
-        t -= nb - I;
-        ++I;
-    }
-    count_tile(p, ve - vb, n, ldp, o, ldo, I, I + t, Pi, Pj);
+    if (flag && (bits & PackTraits<K>::range_mask)) atomicOr(flag, 1u);
 }
 
 __global__ __launch_bounds__(256) void reduce_slices_kernel(const uint32_t* __restrict__ part, uint32_t slices,
                                                             size_t plane, uint32_t n, uint32_t* __restrict__ d,
-                                                            size_t ldd) {
+                                                            size_t ldd, uint32_t accumulate) {
     const uint32_t i = blockIdx.x;
     for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) {
-        uint32_t s = 0;
+        uint32_t s = accumulate ? d[(size_t)i * ldd + j] : 0u;
         for (uint32_t k = 0; k < slices; ++k) s += part[(size_t)k * plane + (size_t)i * n + j];
         d[(size_t)i * ldd + j] = s;
     }
@@ -79,27 +85,46 @@ __global__ __launch_bounds__(256) void count_rankings_kernel(const int32_t* __re
     }
 }
 
+template <uint32_t K>
+void launch_count(uint32_t grid, hipStream_t s, const int32_t* pos, uint32_t v0, uint32_t v1, uint32_t n,
+                  size_t ldp, uint32_t* out, size_t ldo, uint32_t nb, uint32_t slices, uint32_t slice_len,
+                  size_t plane, uint32_t accumulate, uint32_t* flag) {
+    count_kernel<K><<<grid, kCThreads, 0, s>>>(pos, v0, v1, n, ldp, out, ldo, nb, slices, slice_len, plane,
+                                               accumulate, flag);
+}
+
 }  // namespace
 
+// Counter plan: packing width K from n (bytes up to 128 candidates, 16-bit halves up to
+// 32768, int32 above), voter slices for parallelism (about 1024 workgroups, at least 32
+// voters each) capped at the largest slice a K-packed accumulation keeps exact, and passes
+// of at most `budget` bytes of partial planes, each pass adding into d.
 int count_device_impl(DeviceContext* ctx, const int32_t* pos, uint32_t m, uint32_t n, uint32_t* d, size_t ldd,
-                      hipStream_t s, double* kernel_ms) {
+                      hipStream_t s, double* kernel_ms, uint32_t* flag) {
     if (n == 0) return SE_OK;
     const uint32_t nb = (n + kCT - 1) / kCT;
     const uint32_t tiles = n <= kCT ? 1 : nb * (nb + 1) / 2;
-    // voter slices: aim at ~1024 workgroups (4 per CU), at least 32 voters per slice, and
-    // partial planes within 1 GiB
+    const uint32_t K = n <= 128 ? 4u : n <= 32768 ? 2u : 1u;
+    const uint64_t plane_bytes = (uint64_t)n * n * 4;
+    // partial-plane budget; SE_COUNT_PARTIAL_BYTES overrides it (tests use it to force passes)
+    uint64_t budget = 1ull << 30;
+    if (const char* env = std::getenv("SE_COUNT_PARTIAL_BYTES")) budget = std::max<uint64_t>(1, std::strtoull(env, nullptr, 10));
+    const uint64_t fit = std::max<uint64_t>(1, budget / plane_bytes);
     uint64_t want = (1024 + tiles - 1) / tiles;
     want = std::min<uint64_t>(want, std::max<uint32_t>(1, m / 32));
-    want = std::min<uint64_t>(want, std::max<uint64_t>(1, (1ull << 30) / ((uint64_t)n * n * 4)));
-    const uint32_t slices = (uint32_t)std::max<uint64_t>(1, want);
-    const uint32_t slice_len = slices > 1 ? (m + slices - 1) / slices : m;
-    uint32_t* out = d;
-    size_t ldo = ldd, plane = 0;
-    if (slices > 1) {
-        plane = (size_t)n * n;
-        out = static_cast<uint32_t*>(scratch(ctx, kSlotCountPart, plane * slices * sizeof(uint32_t)));
-        if (!out) return SE_ERR_ALLOC;
-        ldo = n;
+    want = std::min<uint64_t>(want, fit);
+    want = std::max<uint64_t>(want, 1);
+    uint64_t slice_len = m ? (m + want - 1) / want : 1;
+    slice_len = round_up(slice_len, K);
+    slice_len = std::min<uint64_t>(slice_len, count_max_slice(K));
+    const uint64_t total_slices = m ? (m + slice_len - 1) / slice_len : 1;
+    const uint64_t per_pass = std::min<uint64_t>(total_slices, fit);
+    const uint64_t passes = (total_slices + per_pass - 1) / per_pass;
+
+    uint32_t* part = nullptr;
+    if (per_pass > 1) {
+        part = static_cast<uint32_t*>(scratch(ctx, kSlotCountPart, plane_bytes * per_pass));
+        if (!part) return SE_ERR_ALLOC;
     }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (kernel_ms) {
@@ -108,11 +133,26 @@ int count_device_impl(DeviceContext* ctx, const int32_t* pos, uint32_t m, uint32
             return SE_ERR_DEVICE;
         (void)hipEventRecord(e0, s);
     }
-    count_kernel<<<tiles * slices, kCThreads, 0, s>>>(pos, m, n, n, out, ldo, nb, slices, slice_len, plane);
-    SE_CHECK_LAUNCH("counter launch failed");
-    if (slices > 1) {
-        reduce_slices_kernel<<<n, 256, 0, s>>>(out, slices, plane, n, d, ldd);
+    for (uint64_t p = 0; p < passes; ++p) {
+        const uint64_t first = p * per_pass;
+        const uint32_t sl = (uint32_t)std::min<uint64_t>(per_pass, total_slices - first);
+        const uint32_t v0 = (uint32_t)std::min<uint64_t>(m, first * slice_len);
+        const uint32_t v1 = (uint32_t)std::min<uint64_t>(m, (first + sl) * slice_len);
+        uint32_t* out = sl > 1 ? part : d;
+        const size_t ldo = sl > 1 ? n : ldd, plane = sl > 1 ? (size_t)n * n : 0;
+        const uint32_t acc = sl > 1 ? 0u : (p > 0 ? 1u : 0u);
+        const uint32_t grid = tiles * sl;
+        if (K == 4)
+            launch_count<4>(grid, s, pos, v0, v1, n, n, out, ldo, nb, sl, (uint32_t)slice_len, plane, acc, flag);
+        else if (K == 2)
+            launch_count<2>(grid, s, pos, v0, v1, n, n, out, ldo, nb, sl, (uint32_t)slice_len, plane, acc, flag);
+        else
+            launch_count<1>(grid, s, pos, v0, v1, n, n, out, ldo, nb, sl, (uint32_t)slice_len, plane, acc, flag);
         SE_CHECK_LAUNCH("counter launch failed");
+        if (sl > 1) {
+            reduce_slices_kernel<<<n, 256, 0, s>>>(part, sl, plane, n, d, ldd, p > 0 ? 1u : 0u);
+            SE_CHECK_LAUNCH("counter launch failed");
+        }
     }
     if (kernel_ms) {
         (void)hipEventRecord(e1, s);
@@ -123,6 +163,13 @@ int count_device_impl(DeviceContext* ctx, const int32_t* pos, uint32_t m, uint32
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
     }
+    return SE_OK;
+}
+
+// Clears a device word (a kernel, not hipMemsetAsync, so the sequence stays capturable).
+int clear_flag(uint32_t* flag, hipStream_t s) {
+    zero_matrix_kernel<<<1, 64, 0, s>>>(flag, 1, 1);
+    SE_CHECK_LAUNCH("counter launch failed");
     return SE_OK;
 }
 

@@ -5,6 +5,21 @@
 // given as position arrays (positions[v][c] = place of candidate c), every voter adds one
 // to d[i][j] exactly when positions[v][i] < positions[v][j]; for a strict order per voter
 // d[j][i] = m - d[i][j] (i != j).
+//
+// Packed counting.  Positions are staged into LDS K voters to a 32-bit word:
+//   K = 4 (n <= 128):   one byte per voter (positions < 128),
+//   K = 2 (n <= 32768): one 16-bit half per voter (positions < 32768),
+//   K = 1 (otherwise):  one int32 per voter, plain compare-and-add.
+// For K > 1 the row side carries a guard bit on top of every lane (0x80 / 0x8000), so
+//   z = (x | guard) - y,   lane value H + x - y  with H = 128 / 32768,
+// lies in [1, 2H) and never borrows from its neighbour; its top bit is [x >= y].  Read as
+// a signed lane, z = x - y + H - 2H [x >= y], so the signed dot product of z with ones
+// (v_dot4c_i32_i8 / v_dot2c_i32_i16) accumulates, over the M voter slots of a slice,
+//   acc = (S_i - S_j) + H M - 2H G,   G = #{slots : x >= y},
+// where S_c is the sum of candidate c's positions over the slice.  Hence
+//   G = (S_i - S_j + H M - acc) / 2H   (exact in uint32 while 2H G < 2^32),
+//   d[i][j] = M - G                    (padding slots hold x = y = 0 and count in G).
+// Two VALU instructions (subtract, dot) per K voter-pairs instead of two per voter-pair.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -14,73 +29,216 @@
 namespace se {
 
 constexpr uint32_t kCT = 128;     // tile side
-constexpr uint32_t kVC = 32;      // voters staged per LDS round
+constexpr uint32_t kVW = 32;      // packed words per candidate staged per LDS round
 constexpr uint32_t kCThreads = 256;
 
-// Lane (ty, tx) owns rows {4 ty + a, 64 + 4 ty + a} and columns {4 tx + b, 64 + 4 tx + b}
-// (a, b < 4): each half-row a wave reads from LDS is 256 contiguous bytes.
-// d tile (I, J), I <= J, of one election: positions pos (m x n, row pitch ldp) -> d (pitch
-// ldd).  Pi/Pj: LDS staging of kVC voters x kCT candidates each.
-__device__ __forceinline__ void count_tile(const int32_t* __restrict__ pos, uint32_t m, uint32_t n, size_t ldp,
-                                           uint32_t* __restrict__ d, size_t ldd, uint32_t I, uint32_t J,
-                                           int32_t (*Pi)[kCT], int32_t (*Pj)[kCT]) {
+// Largest voter slice (slots) a K-packed accumulation stays exact for.
+__host__ __device__ constexpr uint32_t count_max_slice(uint32_t K) {
+    return K == 4 ? (1u << 24) - 4 : K == 2 ? 65534u : 0xFFFFFFFFu;
+}
+// Packing width for an election of n candidates and (per slice) m voters.
+__host__ __device__ constexpr uint32_t count_pack(uint32_t n, uint32_t m) {
+    return n <= 128 ? 4u : (n <= 32768 && m <= count_max_slice(2)) ? 2u : 1u;
+}
+
+template <uint32_t K>
+struct PackTraits;
+template <>
+struct PackTraits<4> {
+    static constexpr uint32_t guard = 0x80808080u, H = 128u, shift = 8u, range_mask = ~0x7Fu;
+};
+template <>
+struct PackTraits<2> {
+    static constexpr uint32_t guard = 0x80008000u, H = 32768u, shift = 16u, range_mask = ~0x7FFFu;
+};
+template <>
+struct PackTraits<1> {
+    static constexpr uint32_t guard = 0u, H = 0u, shift = 0u, range_mask = 0u;
+};
+
+typedef short short2_t __attribute__((ext_vector_type(2)));
+
+// One K-packed word of row side x against column side y.
+template <uint32_t K>
+__device__ __forceinline__ void tally(int32_t& acc, uint32_t x, uint32_t y) {
+    if constexpr (K == 4) {
+        acc = __builtin_amdgcn_sdot4((int)(x - y), 0x01010101, acc, false);
+    } else if constexpr (K == 2) {
+        const uint32_t z = x - y;
+        acc = __builtin_amdgcn_sdot2(__builtin_bit_cast(short2_t, z), short2_t{1, 1}, acc, false);
+    } else {
+        acc += (int32_t)((int32_t)x < (int32_t)y);
+    }
+}
+
+__device__ __forceinline__ void put1(uint32_t* p, uint32_t v, bool accumulate) { *p = accumulate ? *p + v : v; }
+__device__ __forceinline__ void put4(uint32_t* p, uint4 v, bool accumulate) {
+    uint4* q = reinterpret_cast<uint4*>(p);
+    if (accumulate) {
+        const uint4 o = *q;
+        v = make_uint4(o.x + v.x, o.y + v.y, o.z + v.z, o.w + v.w);
+    }
+    *q = v;
+}
+
+// d[i][j] from the accumulator of one cell (see the header comment).
+template <uint32_t K>
+__device__ __forceinline__ uint32_t finish(int32_t acc, uint32_t si, uint32_t sj, uint32_t M) {
+    if constexpr (K == 1) {
+        return (uint32_t)acc;
+    } else {
+        const uint32_t X = si - sj + PackTraits<K>::H * M - (uint32_t)acc;
+        return M - (X >> (PackTraits<K>::shift));
+    }
+}
+
+// Stages words [w0, w0 + kVW) of the slice (voters vb..ve, K per word, zero padded) for
+// rows I0 + [0, ri) into PI (guarded) and columns J0 + [0, rj) into PJ.  Thread t < 128
+// stages row candidate t, thread 128 + t column candidate t; each adds its positions to
+// `s` (the per-candidate sum) and ORs them into `bits` (range check).
+template <uint32_t K>
+__device__ __forceinline__ void stage_words(const int32_t* __restrict__ pos, size_t ldp, uint32_t vb, uint32_t ve,
+                                            uint32_t w0, uint32_t I0, uint32_t ri, uint32_t J0, uint32_t rj,
+                                            uint32_t (*PI)[kCT], uint32_t (*PJ)[kCT], uint32_t& s,
+                                            uint32_t& bits) {
+    const uint32_t t = threadIdx.x, side = t >> 7, c = t & (kCT - 1);
+    const uint32_t base = side ? J0 : I0, r = side ? rj : ri;
+    uint32_t(*P)[kCT] = side ? PJ : PI;
+    const uint32_t guard = side ? 0u : PackTraits<K>::guard;
+    const bool live = c < r;
+    const int32_t* col = pos + base + c;
+#pragma unroll 4
+    for (uint32_t w = 0; w < kVW; ++w) {
+        uint32_t word = 0;
+        if (live) {
+            const uint32_t v0 = vb + (w0 + w) * K;
+#pragma unroll
+            for (uint32_t k = 0; k < K; ++k) {
+                const uint32_t v = v0 + k;
+                const uint32_t p = v < ve ? (uint32_t)col[(size_t)v * ldp] : 0u;
+                if constexpr (K > 1) {
+                    word |= p << (PackTraits<K>::shift * k);
+                    s += p;
+                    bits |= p;
+                } else {
+                    word = p;
+                }
+            }
+        }
+        P[w][c] = word | guard;
+    }
+}
+
+// d tile (I, J), I <= J, full 128 x 128, of one voter slice [vb, ve) of an election with
+// n candidates: positions pos (row pitch ldp) -> d (pitch ldd).  Lane (ty, tx) owns rows
+// {4 ty + a, 64 + 4 ty + a} and columns {4 tx + b, 64 + 4 tx + b} (a, b < 4): per word a
+// wave reads four 16-byte row chunks (broadcast) and 256 contiguous column bytes.
+// PI/PJ: LDS staging of kVW words x kCT candidates; red: 2 x kCT words (may alias PI).
+// With `accumulate` the counts are added to d instead of stored.  Returns the OR of all
+// staged positions (range check of the caller).
+template <uint32_t K>
+__device__ __forceinline__ uint32_t count_tile(const int32_t* __restrict__ pos, uint32_t vb, uint32_t ve,
+                                               uint32_t n, size_t ldp, uint32_t* __restrict__ d, size_t ldd,
+                                               uint32_t I, uint32_t J, uint32_t (*PI)[kCT], uint32_t (*PJ)[kCT],
+                                               uint32_t* red, bool accumulate) {
     const uint32_t I0 = I * kCT, J0 = J * kCT;
     const uint32_t tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const uint32_t ri = min(kCT, n - I0), rj = min(kCT, n - J0);
+    const uint32_t words = (ve - vb + K - 1) / K;
 
-    uint32_t acc[8][8];
+    int32_t acc[8][8];
 #pragma unroll
     for (int a = 0; a < 8; ++a)
 #pragma unroll
         for (int b = 0; b < 8; ++b) acc[a][b] = 0;
+    uint32_t s = 0, bits = 0;
 
-    for (uint32_t v0 = 0; v0 < m; v0 += kVC) {
+    for (uint32_t w0 = 0; w0 < words; w0 += kVW) {
         __syncthreads();
-        for (uint32_t q = threadIdx.x; q < kVC * kCT; q += kCThreads) {
-            uint32_t v = q / kCT, c = q % kCT;
-            bool ok = v0 + v < m;
-            Pi[v][c] = (ok && I0 + c < n) ? pos[(size_t)(v0 + v) * ldp + I0 + c] : 0;
-            Pj[v][c] = (ok && J0 + c < n) ? pos[(size_t)(v0 + v) * ldp + J0 + c] : 0;
-        }
+        stage_words<K>(pos, ldp, vb, ve, w0, I0, ri, J0, rj, PI, PJ, s, bits);
         __syncthreads();
-        const uint32_t vend = min(kVC, m - v0);
-        for (uint32_t v = 0; v < vend; ++v) {
-            const int4 i0 = *reinterpret_cast<const int4*>(&Pi[v][4 * ty]);
-            const int4 i1 = *reinterpret_cast<const int4*>(&Pi[v][64 + 4 * ty]);
-            const int4 j0 = *reinterpret_cast<const int4*>(&Pj[v][4 * tx]);
-            const int4 j1 = *reinterpret_cast<const int4*>(&Pj[v][64 + 4 * tx]);
-            const int32_t pi[8] = {i0.x, i0.y, i0.z, i0.w, i1.x, i1.y, i1.z, i1.w};
-            const int32_t pj[8] = {j0.x, j0.y, j0.z, j0.w, j1.x, j1.y, j1.z, j1.w};
+        const uint32_t wend = min(kVW, words - w0);
+        for (uint32_t w = 0; w < wend; ++w) {
+            const uint4 i0 = *reinterpret_cast<const uint4*>(&PI[w][4 * ty]);
+            const uint4 i1 = *reinterpret_cast<const uint4*>(&PI[w][64 + 4 * ty]);
+            const uint4 j0 = *reinterpret_cast<const uint4*>(&PJ[w][4 * tx]);
+            const uint4 j1 = *reinterpret_cast<const uint4*>(&PJ[w][64 + 4 * tx]);
+            const uint32_t x[8] = {i0.x, i0.y, i0.z, i0.w, i1.x, i1.y, i1.z, i1.w};
+            const uint32_t y[8] = {j0.x, j0.y, j0.z, j0.w, j1.x, j1.y, j1.z, j1.w};
 #pragma unroll
             for (int a = 0; a < 8; ++a)
 #pragma unroll
-                for (int b = 0; b < 8; ++b) acc[a][b] += (uint32_t)(pi[a] < pj[b]);
+                for (int b = 0; b < 8; ++b) tally<K>(acc[a][b], x[a], y[b]);
         }
     }
+    __syncthreads();
+    red[threadIdx.x] = s;  // row sums in red[0, 128), column sums in red[128, 256)
+    __syncthreads();
 
+    const uint32_t M = words * K, mreal = ve - vb;
+    const bool full = ri == kCT && rj == kCT && (ldd & 3) == 0 && ((uintptr_t)d & 15) == 0;
+    uint32_t out[8][8];
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
-        const uint32_t i = I0 + (a < 4 ? 4 * ty + a : 64 + 4 * ty + (a - 4));
-        if (i >= n) continue;
+        const uint32_t li = a < 4 ? 4 * ty + a : 64 + 4 * ty + (a - 4);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
-            const uint32_t j = J0 + (b < 4 ? 4 * tx + b : 64 + 4 * tx + (b - 4));
-            if (j >= n) continue;
-            d[(size_t)i * ldd + j] = acc[a][b];
-            if (I != J) d[(size_t)j * ldd + i] = m - acc[a][b];
+            const uint32_t lj = b < 4 ? 4 * tx + b : 64 + 4 * tx + (b - 4);
+            out[a][b] = finish<K>(acc[a][b], red[li], red[kCT + lj], M);
         }
     }
+    if (full) {
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            const uint32_t i = I0 + (a < 4 ? 4 * ty + a : 64 + 4 * ty + (a - 4));
+            put4(d + (size_t)i * ldd + J0 + 4 * tx, make_uint4(out[a][0], out[a][1], out[a][2], out[a][3]),
+                 accumulate);
+            put4(d + (size_t)i * ldd + J0 + 64 + 4 * tx, make_uint4(out[a][4], out[a][5], out[a][6], out[a][7]),
+                 accumulate);
+        }
+        if (I != J) {
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const uint32_t j = J0 + (b < 4 ? 4 * tx + b : 64 + 4 * tx + (b - 4));
+                put4(d + (size_t)j * ldd + I0 + 4 * ty,
+                     make_uint4(mreal - out[0][b], mreal - out[1][b], mreal - out[2][b], mreal - out[3][b]),
+                     accumulate);
+                put4(d + (size_t)j * ldd + I0 + 64 + 4 * ty,
+                     make_uint4(mreal - out[4][b], mreal - out[5][b], mreal - out[6][b], mreal - out[7][b]),
+                     accumulate);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            const uint32_t li = a < 4 ? 4 * ty + a : 64 + 4 * ty + (a - 4);
+            if (li >= ri) continue;
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const uint32_t lj = b < 4 ? 4 * tx + b : 64 + 4 * tx + (b - 4);
+                if (lj >= rj) continue;
+                put1(d + (size_t)(I0 + li) * ldd + J0 + lj, out[a][b], accumulate);
+                if (I != J) put1(d + (size_t)(J0 + lj) * ldd + I0 + li, mreal - out[a][b], accumulate);
+            }
+        }
+    }
+    return bits;
 }
 
 // Voter-parallel counting of one sub-block of d: rows [I0, I0 + ri) x columns [J0, J0 + rj)
-// (ri, rj <= 128) of an election with n candidates.  The sub-block is cut into 8 x 8 blocks
-// and the 256 lanes into G = 256 / blocks voter groups; group g counts voters g, g + G, ...
-// and the partial blocks are summed through LDS (`red`, at least 256 * 8 words).  On a
-// diagonal sub-block (I0 == J0) only blocks bi <= bj are counted; a strict order per voter
-// gives the lower ones by complement, d[j][i] = m - d[i][j] (the diagonal counts itself: 0).
-// Off-diagonal sub-blocks write d[i][j] and, by complement, d[j][i].
-__device__ __forceinline__ void count_block(const int32_t* __restrict__ pos, uint32_t m, uint32_t n, size_t ldp,
-                                            uint32_t I0, uint32_t J0, uint32_t ri, uint32_t rj,
-                                            uint32_t* __restrict__ d, size_t ldd, int32_t* red) {
+// (ri, rj <= 128) of voter slice [vb, ve).  The sub-block is cut into 8 x 8 blocks and the
+// 256 lanes into G = 256 / blocks word groups; group g counts staged words g, g + G, ...
+// and the partial blocks are summed through LDS (`red`, at least 256 * 8 + 256 words,
+// may alias PI).  On a diagonal sub-block (I0 == J0) only blocks bi <= bj are counted; a
+// strict order per voter gives the lower ones by complement, d[j][i] = m - d[i][j] (the
+// diagonal counts itself: 0).  Off-diagonal sub-blocks write d[i][j] and, by complement,
+// d[j][i].  With `accumulate` the counts are added to d.  Returns the OR of all staged
+// positions.
+template <uint32_t K>
+__device__ __forceinline__ uint32_t count_block(const int32_t* __restrict__ pos, uint32_t vb, uint32_t ve,
+                                                size_t ldp, uint32_t I0, uint32_t J0, uint32_t ri, uint32_t rj,
+                                                uint32_t* __restrict__ d, size_t ldd, uint32_t (*PI)[kCT],
+                                                uint32_t (*PJ)[kCT], uint32_t* red, bool accumulate) {
     const uint32_t t = threadIdx.x;
     const bool diag = I0 == J0;
     const uint32_t nbi = (ri + 7) / 8, nbj = (rj + 7) / 8;
@@ -100,37 +258,47 @@ __device__ __forceinline__ void count_block(const int32_t* __restrict__ pos, uin
         bi = b / nbj;
         bj = b % nbj;
     }
-    uint32_t acc[8][8];
+    const uint32_t words = (ve - vb + K - 1) / K;
+    int32_t acc[8][8];
 #pragma unroll
     for (int a = 0; a < 8; ++a)
 #pragma unroll
         for (int c = 0; c < 8; ++c) acc[a][c] = 0;
-    if (g < G) {
-        for (uint32_t v = g; v < m; v += G) {
-            const int32_t* row = pos + (size_t)v * ldp;
-            int32_t pi[8], pj[8];
+    uint32_t s = 0, bits = 0;
+    for (uint32_t w0 = 0; w0 < words; w0 += kVW) {
+        __syncthreads();
+        stage_words<K>(pos, ldp, vb, ve, w0, I0, ri, J0, rj, PI, PJ, s, bits);
+        __syncthreads();
+        if (g < G) {
+            const uint32_t wend = min(kVW, words - w0);
+            for (uint32_t w = g; w < wend; w += G) {
+                const uint4 i0 = *reinterpret_cast<const uint4*>(&PI[w][8 * bi]);
+                const uint4 i1 = *reinterpret_cast<const uint4*>(&PI[w][8 * bi + 4]);
+                const uint4 j0 = *reinterpret_cast<const uint4*>(&PJ[w][8 * bj]);
+                const uint4 j1 = *reinterpret_cast<const uint4*>(&PJ[w][8 * bj + 4]);
+                const uint32_t x[8] = {i0.x, i0.y, i0.z, i0.w, i1.x, i1.y, i1.z, i1.w};
+                const uint32_t y[8] = {j0.x, j0.y, j0.z, j0.w, j1.x, j1.y, j1.z, j1.w};
 #pragma unroll
-            for (int a = 0; a < 8; ++a) {
-                const uint32_t ci = 8 * bi + a, cj = 8 * bj + a;
-                pi[a] = ci < ri ? row[I0 + ci] : 0;
-                pj[a] = cj < rj ? row[J0 + cj] : 0;
+                for (int a = 0; a < 8; ++a)
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) tally<K>(acc[a][c], x[a], y[c]);
             }
-#pragma unroll
-            for (int a = 0; a < 8; ++a)
-#pragma unroll
-                for (int c = 0; c < 8; ++c) acc[a][c] += (uint32_t)(pi[a] < pj[c]);
         }
     }
+    uint32_t* sums = red + kCThreads * 8;  // row sums [0, 128), column sums [128, 256)
+    __syncthreads();
+    sums[t] = s;
+    const uint32_t M = words * K, mreal = ve - vb;
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
         __syncthreads();
 #pragma unroll
-        for (int c = 0; c < 8; ++c) red[t * 8 + c] = (int32_t)acc[a][c];
+        for (int c = 0; c < 8; ++c) red[t * 8 + c] = (uint32_t)acc[a][c];
         __syncthreads();
         for (uint32_t q = t; q < blocks * 8; q += kCThreads) {
             const uint32_t bb = q / 8, c = q % 8;
             uint32_t sum = 0;
-            for (uint32_t gg = 0; gg < G; ++gg) sum += (uint32_t)red[(gg * blocks + bb) * 8 + c];
+            for (uint32_t gg = 0; gg < G; ++gg) sum += red[(gg * blocks + bb) * 8 + c];
             uint32_t qi, qj;  // block coordinates of bb
             if (diag) {
                 uint32_t r = bb;
@@ -146,22 +314,13 @@ __device__ __forceinline__ void count_block(const int32_t* __restrict__ pos, uin
             }
             const uint32_t li = 8 * qi + a, lj = 8 * qj + c;
             if (li >= ri || lj >= rj) continue;
+            const uint32_t v = finish<K>((int32_t)sum, sums[li], sums[kCT + lj], M);
             const uint32_t i = I0 + li, j = J0 + lj;
-            if (diag && qi == qj) {
-                d[(size_t)i * ldd + j] = sum;  // diagonal block: both triangles counted
-            } else {
-                d[(size_t)i * ldd + j] = sum;
-                d[(size_t)j * ldd + i] = m - sum;
-            }
+            put1(d + (size_t)i * ldd + j, v, accumulate);
+            if (!(diag && qi == qj)) put1(d + (size_t)j * ldd + i, mreal - v, accumulate);  // diagonal block: both triangles counted
         }
     }
-    (void)n;
-}
-
-// Whole election with n <= 128 candidates (one diagonal sub-block).
-__device__ __forceinline__ void count_small(const int32_t* __restrict__ pos, uint32_t m, uint32_t n, size_t ldp,
-                                            uint32_t* __restrict__ d, size_t ldd, int32_t* red) {
-    count_block(pos, m, n, ldp, 0, 0, n, n, d, ldd, red);
+    return bits;
 }
 
 }  // namespace se

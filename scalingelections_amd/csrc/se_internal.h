@@ -33,6 +33,7 @@ enum Slot : int {
     kSlotBatchD = 6,   // batched preference matrices
     kSlotBatchMeta = 7,  // batch metadata / per-launch wins
     kSlotCountPart = 8,  // per-slice partial preference matrices
+    kSlotCountFlag = 9,  // counter range-check flag
     kSlotCount = 10
 };
 

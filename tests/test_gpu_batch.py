@@ -57,6 +57,18 @@ def test_batch_zero_and_single_voter(gpu_lib, oracle_mod):
     _check(oracle_mod, batch.make_workload(C, m, np.ones(5), seed=12), "tiny")
 
 
+def test_batch_many_voters_every_packing(gpu_lib, oracle_mod):
+    """Counter packing per election: bytes (C <= 128), 16-bit halves (C <= 1024 with at
+    most 65534 voters) and one int32 per voter (more voters than a 16-bit slice holds);
+    70000 voters also take the batch's 32-bit closure."""
+    from scalingelections_amd import batch
+
+    C = np.array([100, 200, 200, 9], dtype=np.uint32)
+    m = np.array([70000, 30000, 70000, 70000], dtype=np.uint32)
+    print("batch many voters", C.tolist(), m.tolist(), flush=True)
+    _check(oracle_mod, batch.make_workload(C, m, np.array([0.9, 0.95, 0.97, 0.6]), seed=14), "voters")
+
+
 def test_batch_mixed_sample(gpu_lib, oracle_mod):
     """200 elections from the mixed-size distribution (smaller voter counts); all compared
     batched vs per-launch, 40 against the oracle."""

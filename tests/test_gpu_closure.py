@@ -201,7 +201,8 @@ def test_counter_incomplete_and_duplicate_ballots(gpu_lib, oracle_mod, seed):
     assert np.array_equal(d, oracle_mod.build_prefs(ballots))
 
 
-@pytest.mark.parametrize("m,n", [(1, 1), (0, 5), (3, 129), (257, 64), (1000, 300), (33, 1000)])
+@pytest.mark.parametrize("m,n", [(1, 1), (0, 5), (3, 129), (257, 64), (1000, 300), (33, 1000),
+                                 (5, 127), (7, 128), (2001, 128), (2001, 129), (130, 256), (3, 4000)])
 def test_counter_shapes(gpu_lib, oracle_mod, m, n):
     from scalingelections_amd import count_positions, synth
 
@@ -209,6 +210,46 @@ def test_counter_shapes(gpu_lib, oracle_mod, m, n):
     pos = synth.positions_from_rankings(R) if m else np.zeros((0, n), np.int32)
     d = count_positions(pos)
     assert np.array_equal(d, oracle_mod.build_prefs(list(R)) if m else np.zeros((n, n), np.uint32))
+
+
+def _count_np(pos: np.ndarray) -> np.ndarray:
+    """d[i][j] = #{v : pos[v][i] < pos[v][j]} by direct comparison, in voter chunks."""
+    m, n = pos.shape
+    d = np.zeros((n, n), np.int64)
+    for v0 in range(0, m, 512):
+        P = pos[v0:v0 + 512]
+        d += (P[:, :, None] < P[:, None, :]).sum(0)
+    return d.astype(np.uint32)
+
+
+@pytest.mark.parametrize("budget_planes", [1, 2])
+def test_counter_passes_beyond_16bit_slices(gpu_lib, monkeypatch, budget_planes):
+    """More voters than one 16-bit-packed slice holds (65534): the counter runs several
+    passes that add into d.  A partial-plane budget of 1 or 2 planes forces passes of one
+    slice (direct accumulation) and of two slices (partial planes + accumulating reduce)."""
+    from scalingelections_amd import count_positions
+
+    n, m = 130, 140000
+    rng = np.random.default_rng(11)
+    pos = np.argsort(rng.random((m, n)), axis=1).astype(np.int32)
+    monkeypatch.setenv("SE_COUNT_PARTIAL_BYTES", str(budget_planes * n * n * 4))
+    d = count_positions(pos)
+    assert np.array_equal(d, _count_np(pos))
+    assert np.array_equal(d.astype(np.int64) + d.T, m * (1 - np.eye(n, dtype=np.int64)))
+
+
+@pytest.mark.parametrize("n,bad", [(100, 128), (100, -1), (200, 40000), (200, -5)])
+def test_counter_rejects_positions_outside_the_packed_range(gpu_lib, n, bad):
+    """Positions must be a permutation of 0..n-1; values the packed counter cannot hold
+    (>= 128 for n <= 128, >= 32768 for n <= 32768, negative) are reported, not miscounted."""
+    from scalingelections_amd import _lib, count_positions
+
+    pos = np.tile(np.arange(n, dtype=np.int32), (9, 1))
+    pos[4, 3] = bad
+    with pytest.raises(_lib.SEError, match="permutation"):
+        count_positions(pos)
+    pos[4, 3] = 3
+    assert np.array_equal(count_positions(pos), _count_np(pos))
 
 
 # ---------------------------------------------------------------- ranking

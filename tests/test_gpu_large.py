@@ -71,3 +71,20 @@ def test_c3_32768_closure(gpu_lib, oracle_mod, flags):
         rng = np.random.default_rng(0)
         for src in rng.choice(n, 3, replace=False):
             assert np.array_equal(oracle_mod.widest_row(p0, int(src)), p[src]), int(src)
+
+
+def test_counter_int32_path_above_32768_candidates(gpu_lib):
+    """32769 candidates: positions no longer fit 16 bits, so the counter compares one int32
+    per voter.  Two voters, one the reverse of the other plus a random one; checked in row
+    blocks against direct comparison."""
+    from scalingelections_amd import count_positions
+
+    n = 32769
+    rng = np.random.default_rng(21)
+    pos = np.stack([rng.permutation(n), rng.permutation(n)]).astype(np.int32)
+    print("counter n=32769 m=2", flush=True)
+    d = count_positions(pos)
+    for r0 in range(0, n, 4096):
+        P = pos[:, r0:r0 + 4096]
+        want = (P[:, :, None] < pos[:, None, :]).sum(0)
+        assert np.array_equal(d[r0:r0 + 4096], want), r0
