@@ -197,6 +197,37 @@ __global__ __launch_bounds__(256) void indep_u32_kernel(uint32_t* __restrict__ W
 }
 
 // ---------------------------------------------------------------------------------------
+// 16-bit look-ahead launch: workgroups [0, npanel) are the panel of the pivot block at
+// kb_next (one row chunk + one column chunk each, panel_u16), the others the independent
+// tiles of the pivot block at kb outside macro row/column mnext (the strip launch did
+// those).  The two groups touch disjoint cells: the panel reads and writes only macro
+// row/column mnext (final after the strip launch) and, in workgroup 0, commits the
+// previous closed pivot into the pivot block at kb, which the independent tiles neither
+// read nor store.  Panel workgroups come first, so they are dispatched first.  With
+// npanel = nb and nothing else the kernel is a plain panel launch.
+template <uint32_t T, bool F16>
+__global__ __launch_bounds__(256) void panel_indep_u16_kernel(uint16_t* __restrict__ W, size_t ldw, uint32_t kb,
+                                                              uint32_t mnext, uint32_t nb, uint32_t npanel,
+                                                              uint32_t kb_next, uint32_t* __restrict__ pivot_out,
+                                                              const uint32_t* __restrict__ pivot_prev,
+                                                              uint32_t kb_prev) {
+    static_assert(panel_u16_lds_elems<T>() <= indep_u16_lds_elems<T>(), "panel LDS must fit the tile LDS");
+    __shared__ __align__(16) uint16_t lds[indep_u16_lds_elems<T>()];
+    if (blockIdx.x < npanel) {
+        // the panel is the round's critical path: its waves issue ahead of the tiles' waves
+        // sharing their SIMDs (the tiles fill the issue slots the panel's barriers leave)
+        __builtin_amdgcn_s_setprio(3);
+        panel_u16<T>(W, ldw, kb_next, blockIdx.x, pivot_out, pivot_prev, kb_prev, lds);
+        return;
+    }
+    const uint32_t t = blockIdx.x - npanel, m = nb - 1;
+    uint32_t bi = t / m, bj = t % m;
+    bi += bi >= mnext ? 1u : 0u;
+    bj += bj >= mnext ? 1u : 0u;
+    indep_tile_u16<T, F16>(W, ldw, kb, bi * kMacro, bj * kMacro, lds);
+}
+
+// ---------------------------------------------------------------------------------------
 // Output: out[i][j] = W[i][j] widened to u32, 0 on the diagonal, padding dropped.
 
 template <typename E>
@@ -340,6 +371,61 @@ int run_rounds(DeviceContext* ctx, E* W, size_t ldw, uint32_t n, uint32_t N, uin
     return SE_OK;
 }
 
+
+// Rounds of the 16-bit closure.  With look-ahead, round r is two launches: the strip tiles
+// of the next pivot (macro row/column mnext), then one launch holding the next round's
+// panel and the rest of round r's independent tiles (panel_indep_u16_kernel).  Without
+// look-ahead: all independent tiles, then the next panel.
+template <uint32_t T, bool F16>
+int run_rounds16(uint16_t* W, size_t ldw, uint32_t n, uint32_t N, uint32_t* pivot_buf, hipStream_t s,
+                 EventTimer& tm, bool lookahead, uint32_t* rounds_out) {
+    const uint32_t nb = N / kMacro;
+    const uint32_t rounds = (n + T - 1) / T;
+    auto pivot_of = [&](uint32_t r) { return pivot_buf + (r & 1) * T * T; };
+    if (rounds == 0) {
+        *rounds_out = 0;
+        return SE_OK;
+    }
+    tm.mark(kKindPanel);
+    panel_indep_u16_kernel<T, F16><<<nb, kThreads, 0, s>>>(W, ldw, 0, 0, nb, nb, 0, pivot_of(0), nullptr, 0);
+    tm.mark(kKindPanel);
+    for (uint32_t r = 0; r < rounds; ++r) {
+        if (g_interrupted.load(std::memory_order_relaxed)) {
+            set_error("Stopped by signal");
+            return SE_ERR_INTERRUPTED;
+        }
+        const uint32_t kb = r * T;
+        const bool next = r + 1 < rounds;
+        if (!next) {
+            tm.mark(kKindIndep);
+            launch_indep<uint16_t, T, F16>(W, ldw, kb, 0, 0, nb, s);
+            tm.mark(kKindIndep);
+        } else if (!lookahead) {
+            tm.mark(kKindIndep);
+            launch_indep<uint16_t, T, F16>(W, ldw, kb, 0, 0, nb, s);
+            tm.mark(kKindIndep);
+            tm.mark(kKindPanel);
+            panel_indep_u16_kernel<T, F16>
+                <<<nb, kThreads, 0, s>>>(W, ldw, kb, 0, nb, nb, kb + T, pivot_of(r + 1), pivot_of(r), kb);
+            tm.mark(kKindPanel);
+        } else {
+            const uint32_t mnext = (kb + T) / kMacro;
+            tm.mark(kKindIndep);
+            launch_indep<uint16_t, T, F16>(W, ldw, kb, 1, mnext, nb, s);
+            panel_indep_u16_kernel<T, F16><<<nb + (nb - 1) * (nb - 1), kThreads, 0, s>>>(
+                W, ldw, kb, mnext, nb, nb, kb + T, pivot_of(r + 1), pivot_of(r), kb);
+            tm.mark(kKindIndep);
+        }
+        SE_CHECK_LAUNCH("closure round launch failed");
+    }
+    tm.mark(kKindPanel);
+    commit_pivot_kernel<uint16_t, T><<<1, kThreads, 0, s>>>(W, ldw, (rounds - 1) * T, pivot_of(rounds - 1));
+    tm.mark(kKindPanel);
+    SE_CHECK_LAUNCH("closure round launch failed");
+    *rounds_out = rounds;
+    return SE_OK;
+}
+
 }  // namespace
 
 // Device pipeline: d (device, n x n u32, row stride ld) -> out (device, n x n u32, ldo).
@@ -418,20 +504,26 @@ int closure_device_impl(DeviceContext* ctx, const uint32_t* d, uint32_t n, size_
     int rc;
     uint16_t* W16 = static_cast<uint16_t*>(W);
     uint32_t* W32 = static_cast<uint32_t*>(W);
-    // look-ahead pays once a round's independent work clearly outlasts the pivot/panel
-    // kernel (measured: slower at N = 4096, faster from 8192 on); SE_FLAG_LOOKAHEAD forces it
-    const bool la = !(flags & SE_FLAG_NO_LOOKAHEAD) && (N >= 8192 || (flags & SE_FLAG_LOOKAHEAD));
-    if (la && !ensure_side_stream(ctx)) return SE_ERR_DEVICE;
-    EventTimer tm_side((flags & SE_FLAG_TIME_KERNELS) != 0, ctx->side);
-    if (use16 && f16)
-        rc = T == 64 ? run_rounds<uint16_t, 64, true>(ctx, W16, ldw, n, N, pivot_buf, s, tm, tm_side, la, &rounds)
-                     : run_rounds<uint16_t, 32, true>(ctx, W16, ldw, n, N, pivot_buf, s, tm, tm_side, la, &rounds);
-    else if (use16)
-        rc = T == 64 ? run_rounds<uint16_t, 64>(ctx, W16, ldw, n, N, pivot_buf, s, tm, tm_side, la, &rounds)
-                     : run_rounds<uint16_t, 32>(ctx, W16, ldw, n, N, pivot_buf, s, tm, tm_side, la, &rounds);
-    else
-        rc = T == 64 ? run_rounds<uint32_t, 64>(ctx, W32, ldw, n, N, pivot_buf, s, tm, tm_side, la, &rounds)
-                     : run_rounds<uint32_t, 32>(ctx, W32, ldw, n, N, pivot_buf, s, tm, tm_side, la, &rounds);
+    // look-ahead: the 16-bit paths run the next panel inside the launch of the current
+    // round's remaining tiles (one stream, any size); the 32-bit path overlaps it from a
+    // side stream, which pays once a round's independent work clearly outlasts the panel
+    // kernel (from N = 8192 on).  SE_FLAG_NO_LOOKAHEAD / SE_FLAG_LOOKAHEAD override.
+    const bool no_la = (flags & SE_FLAG_NO_LOOKAHEAD) != 0;
+    const bool la32 = !use16 && !no_la && (N >= 8192 || (flags & SE_FLAG_LOOKAHEAD));
+    if (la32 && !ensure_side_stream(ctx)) return SE_ERR_DEVICE;
+    EventTimer tm_side((flags & SE_FLAG_TIME_KERNELS) != 0 && la32, ctx->side);
+    if (use16) {
+        const bool la = !no_la;
+        if (f16)
+            rc = T == 64 ? run_rounds16<64, true>(W16, ldw, n, N, pivot_buf, s, tm, la, &rounds)
+                         : run_rounds16<32, true>(W16, ldw, n, N, pivot_buf, s, tm, la, &rounds);
+        else
+            rc = T == 64 ? run_rounds16<64, false>(W16, ldw, n, N, pivot_buf, s, tm, la, &rounds)
+                         : run_rounds16<32, false>(W16, ldw, n, N, pivot_buf, s, tm, la, &rounds);
+    } else {
+        rc = T == 64 ? run_rounds<uint32_t, 64>(ctx, W32, ldw, n, N, pivot_buf, s, tm, tm_side, la32, &rounds)
+                     : run_rounds<uint32_t, 32>(ctx, W32, ldw, n, N, pivot_buf, s, tm, tm_side, la32, &rounds);
+    }
     if (rc != SE_OK) return rc;
 
     dim3 go(n);

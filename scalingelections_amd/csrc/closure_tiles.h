@@ -400,4 +400,191 @@ __device__ __forceinline__ void indep_tile_u32(uint32_t* __restrict__ W, size_t 
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// 16-bit panels (one workgroup: the row chunk and the column chunk `chunk` of the pivot
+// block starting at kb).  Every workgroup closes the pivot tile in registers, keeps it as
+// u16 in LDS (pitch T + 2) and runs both chunk products on packed u16 pairs
+// (v_pk_min_u16 / v_pk_max_u16: exact on the integer values and on the f16-ordered bit
+// patterns alike, whose unsigned order is their value order).  LDS use (about 27 KB at
+// T = 64) stays below the independent kernel's, so panel workgroups and independent tiles
+// can share one launch.
+//   P16: T x (T + 2) u16                  closed pivot
+//   S:   T x 128 u16 (row chunk, pitch 128) or T x 136 u16 (column chunk, transposed:
+//        S[k][i] = W[c0 + i][kb + k], pitch 136 spreads consecutive k over the banks)
+//   rowbuf, colbuf: 2 x T u32 each         pivot-closure broadcast
+template <uint32_t T>
+constexpr uint32_t panel_u16_lds_elems() {
+    return T * (T + 2) + T * 136 + 8 * T;
+}
+
+template <uint32_t T>
+__device__ __forceinline__ void close_pivot_u16(const uint16_t* __restrict__ W, size_t ldw, uint32_t kb,
+                                                uint16_t* P16, uint32_t* rowbuf, uint32_t* colbuf) {
+    constexpr uint32_t R = T / 16, PP = T + 2;
+    const uint32_t t = threadIdx.x, ty = t >> 4, tx = t & 15;
+    const uint32_t pr = R * ty, pc = R * tx;
+    uint32_t cell[R][R];
+#pragma unroll
+    for (uint32_t r = 0; r < R; ++r)
+#pragma unroll
+        for (uint32_t c = 0; c < R; ++c) cell[r][c] = W[(size_t)(kb + pr + r) * ldw + kb + pc + c];
+#pragma unroll
+    for (uint32_t k = 0; k < T; ++k) {
+        const uint32_t kr = k % R, buf = (k & 1) * T;
+        if (ty == k / R) {
+#pragma unroll
+            for (uint32_t c = 0; c < R; ++c) rowbuf[buf + pc + c] = cell[kr][c];
+        }
+        if (tx == k / R) {
+#pragma unroll
+            for (uint32_t r = 0; r < R; ++r) colbuf[buf + pr + r] = cell[r][kr];
+        }
+        __syncthreads();
+        uint32_t a[R], b[R];
+#pragma unroll
+        for (uint32_t r = 0; r < R; ++r) a[r] = colbuf[buf + pr + r];
+#pragma unroll
+        for (uint32_t c = 0; c < R; ++c) b[c] = rowbuf[buf + pc + c];
+#pragma unroll
+        for (uint32_t r = 0; r < R; ++r)
+#pragma unroll
+            for (uint32_t c = 0; c < R; ++c) cell[r][c] = max(cell[r][c], min(a[r], b[c]));
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < R; ++r)
+#pragma unroll
+        for (uint32_t c = 0; c < R; ++c) P16[(pr + r) * PP + pc + c] = (uint16_t)cell[r][c];
+    __syncthreads();
+}
+
+// Row chunk (K, [c0, c0 + 128)): new[i][j] = max(old[i][j], max_k min(P[i][k], old[k][j])).
+// Lane (ty, tx): pivot rows R ty + r (r < R), columns c0 + 8 tx .. 8 tx + 7 (4 packed pairs).
+// S holds the chunk's old values (staged by the caller, followed by a barrier).
+template <uint32_t T>
+__device__ __forceinline__ void panel_rows_u16(uint16_t* __restrict__ W, size_t ldw, uint32_t kb, uint32_t c0,
+                                               const uint16_t* P16, const uint16_t* S) {
+    constexpr uint32_t R = T / 16, PP = T + 2;
+    const uint32_t t = threadIdx.x, ty = t >> 4, tx = t & 15;
+    u16x2 acc[R][4];
+#pragma unroll
+    for (uint32_t r = 0; r < R; ++r) {
+        const uint4 v = *reinterpret_cast<const uint4*>(S + (R * ty + r) * kMacro + 8 * tx);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[r][q] = as_u16x2(word(v, q));
+    }
+#pragma unroll 2
+    for (uint32_t k = 0; k < T; k += 2) {
+        const uint4 b4 = *reinterpret_cast<const uint4*>(S + k * kMacro + 8 * tx);
+        const uint4 b5 = *reinterpret_cast<const uint4*>(S + (k + 1) * kMacro + 8 * tx);
+#pragma unroll
+        for (uint32_t r = 0; r < R; ++r) {
+            const u16x2 a2 = as_u16x2(*reinterpret_cast<const uint32_t*>(P16 + (R * ty + r) * PP + k));
+            const u16x2 a0 = a2.xx, a1 = a2.yy;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                acc[r][q] = pk_max(pk_max(acc[r][q], pk_min(a0, as_u16x2(word(b4, q)))),
+                                   pk_min(a1, as_u16x2(word(b5, q))));
+        }
+    }
+    if (c0 + 8 * tx - kb < T) return;  // columns of the pivot block stay untouched
+#pragma unroll
+    for (uint32_t r = 0; r < R; ++r)
+        *reinterpret_cast<uint4*>(W + (size_t)(kb + R * ty + r) * ldw + c0 + 8 * tx) =
+            make_uint4(as_u32(acc[r][0]), as_u32(acc[r][1]), as_u32(acc[r][2]), as_u32(acc[r][3]));
+}
+
+// Column chunk ([c0, c0 + 128), K): new[i][j] = max(old[i][j], max_k min(old[i][k], P[k][j])).
+// Lane (ty, tx): rows c0 + 8 ty + r (r < 8), pivot columns R tx .. R tx + R - 1 (R / 2 pairs).
+template <uint32_t T>
+__device__ __forceinline__ void panel_cols_u16(uint16_t* __restrict__ W, size_t ldw, uint32_t kb, uint32_t c0,
+                                               const uint16_t* P16, uint16_t* S) {
+    constexpr uint32_t R = T / 16, PP = T + 2, SP = 136;
+    static_assert(R % 2 == 0, "column chunk pairs need an even share of pivot columns");
+    const uint32_t t = threadIdx.x, ty = t >> 4, tx = t & 15;
+    __syncthreads();  // S held the row chunk
+    // transposed staging: thread (k, 8-row group) gathers 8 rows of column kb + k
+    for (uint32_t q = t; q < T * 16; q += kThreads) {
+        const uint32_t k = q % T, g = q / T;
+        uint32_t w[4];
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            const uint32_t lo = W[(size_t)(c0 + 8 * g + 2 * h) * ldw + kb + k];
+            const uint32_t hi = W[(size_t)(c0 + 8 * g + 2 * h + 1) * ldw + kb + k];
+            w[h] = lo | (hi << 16);
+        }
+        *reinterpret_cast<uint4*>(S + k * SP + 8 * g) = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+    __syncthreads();
+    u16x2 acc[8][R / 2];
+#pragma unroll
+    for (uint32_t c = 0; c < R; c += 2) {
+        const uint4 v0 = *reinterpret_cast<const uint4*>(S + (R * tx + c) * SP + 8 * ty);
+        const uint4 v1 = *reinterpret_cast<const uint4*>(S + (R * tx + c + 1) * SP + 8 * ty);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const uint32_t x0 = (word(v0, r >> 1) >> (16 * (r & 1))) & 0xFFFFu;
+            const uint32_t x1 = (word(v1, r >> 1) >> (16 * (r & 1))) & 0xFFFFu;
+            acc[r][c / 2] = as_u16x2(x0 | (x1 << 16));
+        }
+    }
+#pragma unroll 2
+    for (uint32_t k = 0; k < T; ++k) {
+        const uint4 a4 = *reinterpret_cast<const uint4*>(S + k * SP + 8 * ty);
+        u16x2 b[R / 2];
+#pragma unroll
+        for (uint32_t c = 0; c < R / 2; ++c)
+            b[c] = as_u16x2(*reinterpret_cast<const uint32_t*>(P16 + k * PP + R * tx + 2 * c));
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const u16x2 a2 = as_u16x2(word(a4, r >> 1));
+            const u16x2 av = (r & 1) ? a2.yy : a2.xx;
+#pragma unroll
+            for (uint32_t c = 0; c < R / 2; ++c) acc[r][c] = pk_max(acc[r][c], pk_min(av, b[c]));
+        }
+    }
+    if (c0 + 8 * ty - kb < T) return;  // rows of the pivot block stay untouched
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        uint16_t* dst = W + (size_t)(c0 + 8 * ty + r) * ldw + kb + R * tx;
+        if constexpr (R == 4) {
+            *reinterpret_cast<uint2*>(dst) = make_uint2(as_u32(acc[r][0]), as_u32(acc[r][1]));
+        } else {
+#pragma unroll
+            for (uint32_t c = 0; c < R / 2; ++c) reinterpret_cast<uint32_t*>(dst)[c] = as_u32(acc[r][c]);
+        }
+    }
+}
+
+// The whole panel workgroup: commits the previous round's closed pivot (workgroup 0, when
+// given), closes this pivot, publishes it (workgroup 0), then row chunk and column chunk
+// `chunk`.  Starts and ends with a barrier.
+template <uint32_t T>
+__device__ __forceinline__ void panel_u16(uint16_t* __restrict__ W, size_t ldw, uint32_t kb, uint32_t chunk,
+                                          uint32_t* __restrict__ pivot_out, const uint32_t* __restrict__ pivot_prev,
+                                          uint32_t kb_prev, uint16_t* lds) {
+    constexpr uint32_t NQ = T * 16 / kThreads;
+    uint16_t* P16 = lds;
+    uint16_t* S = lds + T * (T + 2);
+    uint32_t* rowbuf = reinterpret_cast<uint32_t*>(S + T * 136);
+    uint32_t* colbuf = rowbuf + 2 * T;
+    const uint32_t t = threadIdx.x, c0 = chunk * kMacro;
+    __syncthreads();
+    if (chunk == 0 && pivot_prev != nullptr) {
+        for (uint32_t q = t; q < T * T; q += kThreads)
+            W[(size_t)(kb_prev + q / T) * ldw + kb_prev + q % T] = (uint16_t)pivot_prev[q];
+    }
+    // row chunk into S (not used by the pivot closure, whose barriers publish it)
+#pragma unroll
+    for (uint32_t i = 0; i < NQ; ++i) {
+        const uint32_t q = t + i * kThreads, row = q >> 4, ch = q & 15;
+        *reinterpret_cast<uint4*>(S + row * kMacro + ch * 8) =
+            *reinterpret_cast<const uint4*>(W + (size_t)(kb + row) * ldw + c0 + ch * 8);
+    }
+    close_pivot_u16<T>(W, ldw, kb, P16, rowbuf, colbuf);
+    if (chunk == 0)
+        for (uint32_t q = t; q < T * T; q += kThreads) pivot_out[q] = P16[(q / T) * (T + 2) + q % T];
+    panel_rows_u16<T>(W, ldw, kb, c0, P16, S);
+    panel_cols_u16<T>(W, ldw, kb, c0, P16, S);
+}
+
 }  // namespace se

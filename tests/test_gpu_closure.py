@@ -82,7 +82,7 @@ def test_width_selection_is_exact(gpu_lib, oracle_mod, hi, width):
 @pytest.mark.parametrize("schedule", ["lookahead", "serial"])
 @pytest.mark.parametrize("width", ["f16", "u16", "u32"])
 @pytest.mark.parametrize("pivot", [32, 64])
-@pytest.mark.parametrize("n", [1, 63, 64, 65, 200, 511, 640])
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 200, 511, 640, 1500])
 def test_every_width_and_pivot_depth(gpu_lib, oracle_mod, schedule, width, pivot, n):
     from scalingelections_amd import _lib
 
