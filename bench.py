@@ -231,7 +231,7 @@ def batch_and_serving(do_serve: bool) -> tuple:
     if not do_serve:
         return bres, None
     # ---- serving
-    small_ref = {e: (Rb[ro[e]:ro[e + 1]].copy(), sha(Pb[po[e]:po[e + 1]])) for e in range(w.elections)}
+    small_ref = {e: (Rb[ro[e]:ro[e + 1]].copy(), serve._sha(Pb[po[e]:po[e + 1]])) for e in range(w.elections)}
     del Pb, Pl, Rl
     t0 = time.time()
     large = batch.make_workload(np.array([8192, 8192], np.uint32), np.array([8192, 8192], np.uint32),
@@ -246,8 +246,9 @@ def batch_and_serving(do_serve: bool) -> tuple:
         t_large.append(secs)
     t_large_s = float(np.median(t_large))
     small_rate = w.elections / t_batch
-    capacity = 1.0 / (0.99 / small_rate + 0.01 * t_large_s)
-    rate = 0.6 * capacity
+    large_workers = 2
+    capacity = 1.0 / (0.99 / small_rate + 0.01 * t_large_s / large_workers)
+    rate = 0.5 * capacity
     count = int(rate * 4.0)
     arr = serve.poisson_schedule(rate, count, seed=5)
     rng = np.random.default_rng(5)
@@ -257,7 +258,7 @@ def batch_and_serving(do_serve: bool) -> tuple:
             reqs.append(serve.Request(k, float(arr[k]), large, int(rng.integers(large.elections))))
         else:
             reqs.append(serve.Request(k, float(arr[k]), w, int(rng.integers(w.elections))))
-    srv = serve.Server(large_slots=2, large_max_c=8192, large_max_m=8192)
+    srv = serve.Server(large_slots=large_workers, large_max_c=8192, large_max_m=8192)
     try:
         res = srv.run(reqs)
     finally:
@@ -272,9 +273,9 @@ def batch_and_serving(do_serve: bool) -> tuple:
         "workload": "Poisson arrivals; sizes from the batch mix plus 1% 8192 x 8192 Mallows elections",
         "offered_req_per_s": round(rate, 1),
         "capacity_estimate_req_per_s": round(capacity, 1),
-        "note": "offered load = 60% of the capacity estimated from the batched rate and the device time of "
-                "one 8192 x 8192 election alone (count + closure + ranking); 20k req/s would exceed capacity "
-                "and only measure queue growth",
+        "note": "offered load = 50% of the capacity estimated from the batched rate and the service time of "
+                "one 8192 x 8192 election alone (upload, count, closure, ranking, download, digest) on each "
+                "of 2 large workers; 20k req/s would exceed capacity and only measure queue growth",
         "completed_req_per_s": s["completed_per_s"],
         "requests": s["requests"], "batches": s["batches"], "mean_batch": s["mean_batch"],
         "latency_ms": {k: s[k] for k in ("all", "small", "large") if k in s},

@@ -203,8 +203,11 @@ __global__ __launch_bounds__(256) void indep_u32_kernel(uint32_t* __restrict__ W
 // those).  The two groups touch disjoint cells: the panel reads and writes only macro
 // row/column mnext (final after the strip launch) and, in workgroup 0, commits the
 // previous closed pivot into the pivot block at kb, which the independent tiles neither
-// read nor store.  Panel workgroups come first, so they are dispatched first.  With
-// npanel = nb and nothing else the kernel is a plain panel launch.
+// read nor store.  Panel workgroups come first, so they are dispatched first; npanel is
+// rounded up to a multiple of 8 (the extra workgroups exit) and the tiles are numbered
+// row-major over the full nb x nb grid (strip tiles exit), so tile (bi, bj) lands on XCD
+// bj mod 8 when nb is a multiple of 8: each XCD's L2 serves the B strips of its columns.
+// With a grid of npanel and nothing else the kernel is a plain panel launch.
 template <uint32_t T, bool F16>
 __global__ __launch_bounds__(256) void panel_indep_u16_kernel(uint16_t* __restrict__ W, size_t ldw, uint32_t kb,
                                                               uint32_t mnext, uint32_t nb, uint32_t npanel,
@@ -214,16 +217,16 @@ __global__ __launch_bounds__(256) void panel_indep_u16_kernel(uint16_t* __restri
     static_assert(panel_u16_lds_elems<T>() <= indep_u16_lds_elems<T>(), "panel LDS must fit the tile LDS");
     __shared__ __align__(16) uint16_t lds[indep_u16_lds_elems<T>()];
     if (blockIdx.x < npanel) {
+        if (blockIdx.x >= nb) return;
         // the panel is the round's critical path: its waves issue ahead of the tiles' waves
         // sharing their SIMDs (the tiles fill the issue slots the panel's barriers leave)
         __builtin_amdgcn_s_setprio(3);
         panel_u16<T>(W, ldw, kb_next, blockIdx.x, pivot_out, pivot_prev, kb_prev, lds);
         return;
     }
-    const uint32_t t = blockIdx.x - npanel, m = nb - 1;
-    uint32_t bi = t / m, bj = t % m;
-    bi += bi >= mnext ? 1u : 0u;
-    bj += bj >= mnext ? 1u : 0u;
+    const uint32_t t = blockIdx.x - npanel;
+    const uint32_t bi = t / nb, bj = t % nb;
+    if (bi == mnext || bj == mnext) return;  // strip tiles ran in the strip launch
     indep_tile_u16<T, F16>(W, ldw, kb, bi * kMacro, bj * kMacro, lds);
 }
 
@@ -386,8 +389,9 @@ int run_rounds16(uint16_t* W, size_t ldw, uint32_t n, uint32_t N, uint32_t* pivo
         *rounds_out = 0;
         return SE_OK;
     }
+    const uint32_t npanel = (nb + 7) / 8 * 8;
     tm.mark(kKindPanel);
-    panel_indep_u16_kernel<T, F16><<<nb, kThreads, 0, s>>>(W, ldw, 0, 0, nb, nb, 0, pivot_of(0), nullptr, 0);
+    panel_indep_u16_kernel<T, F16><<<npanel, kThreads, 0, s>>>(W, ldw, 0, 0, nb, npanel, 0, pivot_of(0), nullptr, 0);
     tm.mark(kKindPanel);
     for (uint32_t r = 0; r < rounds; ++r) {
         if (g_interrupted.load(std::memory_order_relaxed)) {
@@ -406,14 +410,14 @@ int run_rounds16(uint16_t* W, size_t ldw, uint32_t n, uint32_t N, uint32_t* pivo
             tm.mark(kKindIndep);
             tm.mark(kKindPanel);
             panel_indep_u16_kernel<T, F16>
-                <<<nb, kThreads, 0, s>>>(W, ldw, kb, 0, nb, nb, kb + T, pivot_of(r + 1), pivot_of(r), kb);
+                <<<npanel, kThreads, 0, s>>>(W, ldw, kb, 0, nb, npanel, kb + T, pivot_of(r + 1), pivot_of(r), kb);
             tm.mark(kKindPanel);
         } else {
             const uint32_t mnext = (kb + T) / kMacro;
             tm.mark(kKindIndep);
             launch_indep<uint16_t, T, F16>(W, ldw, kb, 1, mnext, nb, s);
-            panel_indep_u16_kernel<T, F16><<<nb + (nb - 1) * (nb - 1), kThreads, 0, s>>>(
-                W, ldw, kb, mnext, nb, nb, kb + T, pivot_of(r + 1), pivot_of(r), kb);
+            panel_indep_u16_kernel<T, F16><<<npanel + nb * nb, kThreads, 0, s>>>(
+                W, ldw, kb, mnext, nb, npanel, kb + T, pivot_of(r + 1), pivot_of(r), kb);
             tm.mark(kKindIndep);
         }
         SE_CHECK_LAUNCH("closure round launch failed");

@@ -8,9 +8,9 @@ strongest paths and ranking.  The server:
   * runs small elections (<= 1024 candidates) in batches: every pass of the loop packs all
     admitted small requests into one ``se_batch_device`` call on the batch stream; requests
     that arrive while a batch runs join the next one;
-  * runs large elections on a second stream from a worker thread through the
-    single-election entry points (counter, closure, ranking), so they overlap with the
-    batches on the device;
+  * runs large elections from worker threads (one per large slot, each with its own
+    stream) through the single-election entry points (counter, closure, ranking), so they
+    overlap with the batches and with each other on the device;
   * keeps inputs and results in device memory (preallocated arenas and slots, no per-request
     hipMalloc) and copies back the ranking, plus the strongest-path matrix when asked.
 
@@ -71,8 +71,18 @@ def poisson_schedule(rate: float, count: int, seed: int) -> np.ndarray:
     return np.cumsum(rng.exponential(1.0 / rate, count))
 
 
+try:  # 128-bit xxh3 digests strongest-path matrices at memory speed; sha256 otherwise
+    import xxhash as _xxhash
+except ImportError:  # pragma: no cover - xxhash ships with this environment
+    _xxhash = None
+
+
 def _sha(a: np.ndarray) -> str:
-    return hashlib.sha256(np.ascontiguousarray(a, dtype="<u4").tobytes()).hexdigest()
+    """Digest of a strongest-path matrix (little-endian uint32 bytes)."""
+    buf = np.ascontiguousarray(a, dtype="<u4")
+    if _xxhash is not None:
+        return "xxh3_128:" + _xxhash.xxh3_128_hexdigest(memoryview(buf).cast("B"))
+    return "sha256:" + hashlib.sha256(buf.tobytes()).hexdigest()
 
 
 @dataclass
@@ -108,7 +118,7 @@ class Server:
         self.budget = int(min(budget_bytes or free * 0.6, free * 0.8))
         self.keep_paths = keep_paths
         self.s_small = devmem.Stream(nonblocking=True)
-        self.s_large = devmem.Stream(nonblocking=True)
+        self.s_large = [devmem.Stream(nonblocking=True) for _ in range(max(1, large_slots))]
         # small-batch arena: positions | paths | ranking
         self.arena_pos = devmem.DeviceBuffer(small_arena_bytes)
         self.arena_paths = devmem.DeviceBuffer(small_arena_bytes // 2)
@@ -116,17 +126,18 @@ class Server:
         # large-request slots
         self.large_cap = (large_max_c, large_max_m)
         self.slots = queue.Queue()
-        for _ in range(large_slots):
+        for i in range(max(1, large_slots)):
             self.slots.put({
                 "pos": devmem.DeviceBuffer(large_max_c * large_max_m * 4),
                 "d": devmem.DeviceBuffer(large_max_c * large_max_c * 4),
                 "p": devmem.DeviceBuffer(large_max_c * large_max_c * 4),
+                "stream": self.s_large[i],
             })
         self._lock = threading.Lock()
         self._inflight = 0
 
     def close(self):
-        for s in (self.s_small, self.s_large):
+        for s in [self.s_small] + self.s_large:
             s.close()
 
     # ------------------------------------------------------------------ accounting
@@ -146,8 +157,9 @@ class Server:
         L = _lib.lib()
         C, m = r.C, r.m
         pos = r.pool.election_positions(r.index)
-        slot["pos"].upload_async(pos, self.s_large)
-        sh = ctypes.c_void_p(self.s_large.handle)
+        stream = slot["stream"]
+        slot["pos"].upload_async(pos, stream)
+        sh = ctypes.c_void_p(stream.handle)
         _lib.check(L.se_count_device(slot["pos"].ptr, m, C, slot["d"].ptr, C, sh, None))
         _lib.check(L.se_closure_device(slot["d"].ptr, C, C, slot["p"].ptr, C, 0, sh, None))
         wins = np.empty(C, np.uint32)
@@ -157,8 +169,8 @@ class Server:
         r.ranking = ranking
         if self.keep_paths:
             p = np.empty((C, C), np.uint32)
-            slot["p"].download_async(p, self.s_large)
-            self.s_large.synchronize()
+            slot["p"].download_async(p, stream)
+            stream.synchronize()
             r.paths_sha256 = _sha(p)
 
     def _large_worker(self, q: "queue.Queue", t0: float):
@@ -219,8 +231,10 @@ class Server:
         N = len(reqs)
         lq: "queue.Queue" = queue.Queue()
         t0 = time.perf_counter()
-        worker = threading.Thread(target=self._large_worker, args=(lq, t0), daemon=True)
-        worker.start()
+        workers = [threading.Thread(target=self._large_worker, args=(lq, t0), daemon=True)
+                   for _ in range(len(self.s_large))]
+        for wk in workers:
+            wk.start()
         small, large = deque(), deque()
         i = 0
         batches, sizes = 0, []
@@ -264,33 +278,39 @@ class Server:
                 nxt = reqs[i].arrival - (time.perf_counter() - t0) if i < N else idle_sleep
                 time.sleep(max(0.0, min(idle_sleep, nxt)))
         finally:
-            lq.put(None)
-            worker.join()
+            for _ in workers:
+                lq.put(None)
+            for wk in workers:
+                wk.join()
         wall = max(r.done for r in reqs) if reqs else 0.0
         return ServeResult(reqs, wall, batches, sizes)
 
 
 def solo_result(pool: Workload, index: int, stream: Optional[devmem.Stream] = None, timed: bool = False):
     """The election processed alone through the single-election entry points:
-    (ranking, sha256 of the strongest-path matrix[, seconds from resident positions to
-    ranking on the host])."""
+    (ranking, digest of the strongest-path matrix[, seconds of the whole service the server
+    gives a large request: upload, count, closure, ranking, download, digest])."""
     L = _lib.lib()
     C, m = int(pool.C[index]), int(pool.m[index])
-    pos = devmem.DeviceBuffer.from_array(pool.election_positions(index))
+    host_pos = pool.election_positions(index)
+    pos = devmem.DeviceBuffer(max(4, host_pos.nbytes))
     d = devmem.DeviceBuffer(C * C * 4)
     p = devmem.DeviceBuffer(C * C * 4)
     sh = ctypes.c_void_p(stream.handle) if stream else None
     devmem.device_synchronize()
     t0 = time.perf_counter()
+    if host_pos.size:
+        pos.upload(host_pos)
     _lib.check(L.se_count_device(pos.ptr, m, C, d.ptr, C, sh, None))
     _lib.check(L.se_closure_device(d.ptr, C, C, p.ptr, C, 0, sh, None))
     wins = np.empty(C, np.uint32)
     ranking = np.empty(C, np.uint32)
     _lib.check(L.se_rank_device(p.ptr, C, C, wins.ctypes.data, ranking.ctypes.data, sh))
-    elapsed = time.perf_counter() - t0
     out = np.empty((C, C), np.uint32)
     devmem.device_synchronize()
     p.download(out)
+    digest = _sha(out)
+    elapsed = time.perf_counter() - t0
     if timed:
-        return ranking, _sha(out), elapsed
-    return ranking, _sha(out)
+        return ranking, digest, elapsed
+    return ranking, digest
