@@ -97,7 +97,7 @@ def recorded_traffic(mode: str, n: int):
     FETCH_SIZE / WRITE_SIZE summary (counters need their own profiler runs)."""
     if mode != "f16" or n != 32768:
         return None
-    path = os.path.join(ROOT, "profiles", "indep_f16_closure32768_hbm_traffic.md")
+    path = os.path.join(ROOT, "profiles", "panel_indep_f16_closure32768_hbm_traffic.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -409,7 +409,9 @@ def main() -> int:
             "parallelism": "single GPU",
         },
         "roofline": {
-            "kernel": f"indep_u{width}_kernel (independent tiles, {mode} instruction mix)",
+            "kernel": (f"panel_indep_u16_kernel + indep_u16_kernel (independent tiles with the next panel, "
+                       f"{mode} instruction mix)" if width == 16 else
+                       "indep_u32_kernel (independent tiles, u32 instruction mix)"),
             "bound": "valu",
             "achieved": round(achieved, 3),
             "peak": round(peak, 2),
@@ -417,7 +419,7 @@ def main() -> int:
             "frac": round(achieved / peak, 4) if peak else None,
             "traffic": recorded_traffic(mode, n),
             "traffic_note": "HBM bytes per round (both launches) from rocprofv3 FETCH_SIZE/WRITE_SIZE scaled by a "
-                            "1 GiB copy, profiles/indep_f16_closure32768_hbm_traffic.md; algorithmic 4.295e9 "
+                            "1 GiB copy, profiles/panel_indep_f16_closure32768_hbm_traffic.md; algorithmic 4.295e9 "
                             "(read + write the 16-bit matrix once)",
             "peak_basis": "256 CUs x 64 lanes x 2.4 GHz VALU issue x cell updates per lane-instruction "
                           "(f16-ordered 4/3, packed u16 1, u32 2/3)",
