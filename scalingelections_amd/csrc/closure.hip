@@ -32,38 +32,71 @@
 namespace se {
 
 // ---------------------------------------------------------------------------------------
-// p0 init.  64 x 64 tiles; the transposed operand d[j][i] goes through LDS so both reads
-// are coalesced.
+// p0 init over 64 x 64 tile pairs {(I, J), (J, I)} with I <= J: each pair reads both blocks
+// of d once (row-major, coalesced; the (J, I) block goes through LDS transposed) and
+// handles both orientations, so d is read once per kernel.  Pairs are numbered row-major
+// over the upper triangle of the nt x nt tile grid and visited grid-stride.
 
 constexpr uint32_t kInitTile = 64;
 
+// Pair t -> (I, J), I <= J, of the row-major upper triangle of an nt x nt grid.
+__device__ __forceinline__ void upper_pair(uint32_t t, uint32_t nt, uint32_t* I, uint32_t* J) {
+    // rows before I hold I nt - I (I - 1) / 2 pairs; solve for I, then correct by +-1
+    const double b = 2.0 * nt + 1.0;
+    int64_t i = (int64_t)((b - sqrt(b * b - 8.0 * (double)t)) / 2.0);
+    auto first = [&](int64_t r) { return (uint64_t)(r * (int64_t)nt - r * (r - 1) / 2); };
+    if (i < 0) i = 0;
+    while (i > 0 && first(i) > t) --i;
+    while (i + 1 < (int64_t)nt && first(i + 1) <= t) ++i;
+    *I = (uint32_t)i;
+    *J = (uint32_t)(i + (t - first(i)));
+}
+
+// Loads d block (I, J) into registers (row a = ty + 4 q) and block (J, I) transposed into
+// tr (tr[b][a] = d[J0 + b][I0 + a]); zeros outside n x n.
+__device__ __forceinline__ void load_pair(const uint32_t* __restrict__ d, uint32_t n, size_t ld, uint32_t I0,
+                                          uint32_t J0, uint32_t (*tr)[kInitTile + 1], uint32_t* v) {
+    const uint32_t tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+#pragma unroll
+    for (uint32_t q = 0; q < kInitTile / 4; ++q) {
+        const uint32_t a = ty + 4 * q;
+        const uint32_t i = I0 + a, j = J0 + tx;
+        v[q] = (i < n && j < n) ? d[(size_t)i * ld + j] : 0u;
+        const uint32_t r = J0 + a, c = I0 + tx;
+        tr[tx][a] = (r < n && c < n) ? d[(size_t)r * ld + c] : 0u;  // tr[c - I0][r - J0]
+    }
+}
+
 // Range of p0 over the n x n domain: out[0] = max (atomicMax), out[1] = smallest nonzero
 // value (atomicMin, stays 0xFFFFFFFF when p0 is all zero).  Decides the 16-bit eligibility
-// and whether the values fit the f16-ordered range.
+// and whether the values fit the f16-ordered range.  A pair (i, j), i != j, contributes
+// max(d[i][j], d[j][i]) when the two differ (that larger count is a nonzero p0 entry).
 __global__ __launch_bounds__(256) void p0_max_kernel(const uint32_t* __restrict__ d, uint32_t n,
-                                                     size_t ld, uint32_t* __restrict__ out) {
+                                                     size_t ld, uint32_t nt, uint32_t* __restrict__ out) {
     __shared__ uint32_t tr[kInitTile][kInitTile + 1];
     __shared__ uint32_t red[kThreads / 64], redmin[kThreads / 64];
-    const uint32_t I0 = blockIdx.y * kInitTile, J0 = blockIdx.x * kInitTile;
     const uint32_t tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    // tr[b][a] = d[J0 + b][I0 + a]
-    for (uint32_t b = ty; b < kInitTile; b += 4) {
-        uint32_t r = J0 + b, c = I0 + tx;
-        tr[b][tx] = (r < n && c < n) ? d[(size_t)r * ld + c] : 0u;
-    }
-    __syncthreads();
+    const uint32_t pairs = nt * (nt + 1) / 2;
     uint32_t m = 0, mn = 0xFFFFFFFFu;
-    for (uint32_t a = ty; a < kInitTile; a += 4) {
-        uint32_t i = I0 + a, j = J0 + tx;
-        if (i < n && j < n && i != j) {
-            uint32_t v = d[(size_t)i * ld + j];
-            if (v > tr[tx][a]) {  // v > d[j][i] >= 0, so v is a nonzero p0 value
-                m = max(m, v);
-                mn = min(mn, v);
+    for (uint32_t t = blockIdx.x; t < pairs; t += gridDim.x) {
+        uint32_t I, J;
+        upper_pair(t, nt, &I, &J);
+        const uint32_t I0 = I * kInitTile, J0 = J * kInitTile;
+        uint32_t v[kInitTile / 4];
+        __syncthreads();
+        load_pair(d, n, ld, I0, J0, tr, v);
+        __syncthreads();
+#pragma unroll
+        for (uint32_t q = 0; q < kInitTile / 4; ++q) {
+            const uint32_t a = ty + 4 * q;
+            const uint32_t x = v[q], y = tr[a][tx];  // d[i][j], d[j][i] with i = I0 + a, j = J0 + tx
+            if (I0 + a != J0 + tx && x != y) {
+                const uint32_t w = max(x, y);
+                m = max(m, w);
+                mn = min(mn, w);
             }
         }
     }
-    // wave reduction then block reduction
     for (int off = 32; off > 0; off >>= 1) {
         m = max(m, (uint32_t)__shfl_xor((int)m, off, 64));
         mn = min(mn, (uint32_t)__shfl_xor((int)mn, off, 64));
@@ -81,29 +114,45 @@ __global__ __launch_bounds__(256) void p0_max_kernel(const uint32_t* __restrict_
     }
 }
 
-// Writes p0 into the padded work matrix W (N x N, row pitch ldw, element E); nonzero
-// values are stored as v + bias (bias = 0, or the f16-ordered offset 0x0400 - min).
+// Writes p0 into the padded work matrix W (N x N, row pitch ldw, element E), both
+// orientations of each tile pair; nonzero values are stored as v + bias (bias = 0, or the
+// f16-ordered offset 0x0400 - min).  Block (J, I) is assembled transposed in LDS, then
+// written row-major.
 template <typename E>
 __global__ __launch_bounds__(256) void p0_write_kernel(const uint32_t* __restrict__ d, uint32_t n,
-                                                       size_t ld, E* __restrict__ W, uint32_t N,
-                                                       size_t ldw, uint32_t bias) {
+                                                       size_t ld, E* __restrict__ W, size_t ldw, uint32_t nt,
+                                                       uint32_t bias) {
     __shared__ uint32_t tr[kInitTile][kInitTile + 1];
-    const uint32_t I0 = blockIdx.y * kInitTile, J0 = blockIdx.x * kInitTile;
     const uint32_t tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    for (uint32_t b = ty; b < kInitTile; b += 4) {
-        uint32_t r = J0 + b, c = I0 + tx;
-        tr[b][tx] = (r < n && c < n) ? d[(size_t)r * ld + c] : 0u;
-    }
-    __syncthreads();
-    for (uint32_t a = ty; a < kInitTile; a += 4) {
-        uint32_t i = I0 + a, j = J0 + tx;
-        if (i >= N || j >= N) continue;
-        uint32_t v = 0;
-        if (i < n && j < n && i != j) {
-            uint32_t x = d[(size_t)i * ld + j];
-            if (x > tr[tx][a]) v = x + bias;
+    const uint32_t pairs = nt * (nt + 1) / 2;
+    for (uint32_t t = blockIdx.x; t < pairs; t += gridDim.x) {
+        uint32_t I, J;
+        upper_pair(t, nt, &I, &J);
+        const uint32_t I0 = I * kInitTile, J0 = J * kInitTile;
+        uint32_t v[kInitTile / 4];
+        __syncthreads();
+        load_pair(d, n, ld, I0, J0, tr, v);
+        __syncthreads();
+        // (I, J) orientation straight out; (J, I) values go back into tr transposed
+#pragma unroll
+        for (uint32_t q = 0; q < kInitTile / 4; ++q) {
+            const uint32_t a = ty + 4 * q;
+            const uint32_t i = I0 + a, j = J0 + tx;
+            const uint32_t x = v[q], y = tr[a][tx];
+            const bool off = i != j && i < n && j < n;
+            W[(size_t)i * ldw + j] = (E)((off && x > y) ? x + bias : 0u);
+            v[q] = (off && y > x) ? y + bias : 0u;  // p0[j][i]
         }
-        W[(size_t)i * ldw + j] = (E)v;
+        if (I == J) continue;  // the diagonal pair wrote both orientations above
+        __syncthreads();
+#pragma unroll
+        for (uint32_t q = 0; q < kInitTile / 4; ++q) tr[tx][ty + 4 * q] = v[q];  // tr[j - J0][i - I0]
+        __syncthreads();
+#pragma unroll
+        for (uint32_t q = 0; q < kInitTile / 4; ++q) {
+            const uint32_t b = ty + 4 * q;  // row J0 + b, column I0 + tx
+            W[(size_t)(J0 + b) * ldw + I0 + tx] = (E)tr[b][tx];
+        }
     }
 }
 
@@ -292,6 +341,12 @@ struct EventTimer {
 
 enum { kKindInit = 0, kKindDiag = 1, kKindPanel = 2, kKindIndep = 3, kKindOut = 4 };
 
+// Workgroups of the p0 passes: one per tile pair up to 4096 (16 per CU), grid-stride beyond.
+uint32_t init_grid(uint32_t nt) {
+    const uint64_t pairs = (uint64_t)nt * (nt + 1) / 2;
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(pairs, 4096));
+}
+
 bool ensure_side_stream(DeviceContext* ctx) {
     if (ctx->side) return true;
     if (!hip_ok(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking), "Failed to create a HIP stream") ||
@@ -464,8 +519,8 @@ int closure_device_impl(DeviceContext* ctx, const uint32_t* d, uint32_t n, size_
         uint32_t range[2] = {0, 0};
         tm.mark(kKindInit);
         init_range_kernel<<<1, 64, 0, s>>>(dmax);
-        dim3 g((n + kInitTile - 1) / kInitTile, (n + kInitTile - 1) / kInitTile);
-        p0_max_kernel<<<g, kThreads, 0, s>>>(d, n, ld, dmax);
+        const uint32_t ntn = (n + kInitTile - 1) / kInitTile;
+        p0_max_kernel<<<init_grid(ntn), kThreads, 0, s>>>(d, n, ld, ntn, dmax);
         tm.mark(kKindInit);
         SE_CHECK_LAUNCH("p0 init launch failed");
         if (!hip_ok(hipMemcpyAsync(range, dmax, sizeof(range), hipMemcpyDeviceToHost, s),
@@ -488,12 +543,13 @@ int closure_device_impl(DeviceContext* ctx, const uint32_t* d, uint32_t n, size_
     void* W = scratch(ctx, kSlotWork, (size_t)N * ldw * esz);
     if (!W) return SE_ERR_ALLOC;
 
-    dim3 gi(N / kInitTile, N / kInitTile);
+    const uint32_t ntN = N / kInitTile;
     tm.mark(kKindInit);
     if (use16)
-        p0_write_kernel<uint16_t><<<gi, kThreads, 0, s>>>(d, n, ld, static_cast<uint16_t*>(W), N, ldw, bias);
+        p0_write_kernel<uint16_t><<<init_grid(ntN), kThreads, 0, s>>>(d, n, ld, static_cast<uint16_t*>(W), ldw, ntN,
+                                                                     bias);
     else
-        p0_write_kernel<uint32_t><<<gi, kThreads, 0, s>>>(d, n, ld, static_cast<uint32_t*>(W), N, ldw, 0u);
+        p0_write_kernel<uint32_t><<<init_grid(ntN), kThreads, 0, s>>>(d, n, ld, static_cast<uint32_t*>(W), ldw, ntN, 0u);
     tm.mark(kKindInit);
     SE_CHECK_LAUNCH("p0 init launch failed");
 
