@@ -71,6 +71,29 @@ __device__ __forceinline__ void tally(int32_t& acc, uint32_t x, uint32_t y) {
     }
 }
 
+// One row of 8 cells: the 8 subtractions first, then the 8 dot products, so no dot waits
+// on the instruction just issued (measured 2-3% faster than alternating them).
+template <uint32_t K>
+__device__ __forceinline__ void tally_row(int32_t* acc, uint32_t x, const uint32_t* y) {
+    if constexpr (K == 1) {
+#pragma unroll
+        for (int b = 0; b < 8; ++b) tally<1>(acc[b], x, y[b]);
+    } else {
+        uint32_t z[8];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) z[b] = x - y[b];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            if constexpr (K == 4)
+                acc[b] = __builtin_amdgcn_sdot4((int)z[b], 0x01010101, acc[b], false);
+            else
+                acc[b] = __builtin_amdgcn_sdot2(__builtin_bit_cast(short2_t, z[b]), short2_t{1, 1}, acc[b], false);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
 __device__ __forceinline__ void put1(uint32_t* p, uint32_t v, bool accumulate) { *p = accumulate ? *p + v : v; }
 __device__ __forceinline__ void put4(uint32_t* p, uint4 v, bool accumulate) {
     uint4* q = reinterpret_cast<uint4*>(p);
@@ -166,9 +189,7 @@ __device__ __forceinline__ uint32_t count_tile(const int32_t* __restrict__ pos, 
             const uint32_t x[8] = {i0.x, i0.y, i0.z, i0.w, i1.x, i1.y, i1.z, i1.w};
             const uint32_t y[8] = {j0.x, j0.y, j0.z, j0.w, j1.x, j1.y, j1.z, j1.w};
 #pragma unroll
-            for (int a = 0; a < 8; ++a)
-#pragma unroll
-                for (int b = 0; b < 8; ++b) tally<K>(acc[a][b], x[a], y[b]);
+            for (int a = 0; a < 8; ++a) tally_row<K>(acc[a], x[a], y);
         }
     }
     __syncthreads();
@@ -279,9 +300,7 @@ __device__ __forceinline__ uint32_t count_block(const int32_t* __restrict__ pos,
                 const uint32_t x[8] = {i0.x, i0.y, i0.z, i0.w, i1.x, i1.y, i1.z, i1.w};
                 const uint32_t y[8] = {j0.x, j0.y, j0.z, j0.w, j1.x, j1.y, j1.z, j1.w};
 #pragma unroll
-                for (int a = 0; a < 8; ++a)
-#pragma unroll
-                    for (int c = 0; c < 8; ++c) tally<K>(acc[a][c], x[a], y[c]);
+                for (int a = 0; a < 8; ++a) tally_row<K>(acc[a], x[a], y);
             }
         }
     }
