@@ -190,9 +190,11 @@ constexpr uint32_t indep_u16_lds_elems() { return kMacro * (T + 8) + T * kMacro;
 // [0x0400, 0x7C00] after an order-preserving offset), whose float order equals their
 // unsigned order; IEEE-754-2019 minimum/maximum return one operand unchanged, so two pivot
 // steps fold into 2 v_pk_minimum3_f16 + 1 v_pk_maximum3_f16 (0.75 VALU per update).
-template <uint32_t T, bool F16 = false>
-__device__ __forceinline__ void indep_tile_u16(uint16_t* __restrict__ W, size_t ldw, uint32_t kb, uint32_t I0,
-                                               uint32_t J0, uint16_t* __restrict__ lds) {
+// MASKED = false: the tile holds no pivot row or column (almost every tile), so loads and
+// stores need no per-row checks.
+template <uint32_t T, bool F16, bool MASKED>
+__device__ __forceinline__ void indep_tile_u16_impl(uint16_t* __restrict__ W, size_t ldw, uint32_t kb,
+                                                    uint32_t I0, uint32_t J0, uint16_t* __restrict__ lds) {
     constexpr uint32_t AP = T + 8;  // A strip pitch (u16), keeps 16-byte alignment
     uint16_t* As = lds;
     uint16_t* Bs = lds + kMacro * AP;
@@ -203,11 +205,16 @@ __device__ __forceinline__ void indep_tile_u16(uint16_t* __restrict__ W, size_t 
     // written concurrently by the next round's panel kernel)
     uint4 c[8];
     const uint16_t* cp = W + (size_t)(I0 + ty) * ldw + J0 + tx * 8;
-    const bool col_in_pivot = J0 + tx * 8 - kb < T;
+    const bool col_in_pivot = MASKED && J0 + tx * 8 - kb < T;
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
-        c[r] = make_uint4(0, 0, 0, 0);
-        if (!col_in_pivot && I0 + ty + 16 * r - kb >= T) c[r] = *reinterpret_cast<const uint4*>(cp + (size_t)(16 * r) * ldw);
+        if constexpr (MASKED) {
+            c[r] = make_uint4(0, 0, 0, 0);
+            if (!col_in_pivot && I0 + ty + 16 * r - kb >= T)
+                c[r] = *reinterpret_cast<const uint4*>(cp + (size_t)(16 * r) * ldw);
+        } else {
+            c[r] = *reinterpret_cast<const uint4*>(cp + (size_t)(16 * r) * ldw);
+        }
     }
 
     // strips; cells of the pivot block (K, K) only feed pivot-row/column cells, which are
@@ -216,13 +223,15 @@ __device__ __forceinline__ void indep_tile_u16(uint16_t* __restrict__ W, size_t 
     for (uint32_t q = t; q < kMacro * (T / 8); q += kThreads) {
         uint32_t row = q / (T / 8), ch = q % (T / 8);
         uint4 v = make_uint4(0, 0, 0, 0);
-        if (I0 + row - kb >= T) v = *reinterpret_cast<const uint4*>(W + (size_t)(I0 + row) * ldw + kb + ch * 8);
+        if (!MASKED || I0 + row - kb >= T)
+            v = *reinterpret_cast<const uint4*>(W + (size_t)(I0 + row) * ldw + kb + ch * 8);
         *reinterpret_cast<uint4*>(As + row * AP + ch * 8) = v;
     }
     for (uint32_t q = t; q < T * 16; q += kThreads) {
         uint32_t row = q >> 4, ch = q & 15;
         uint4 v = make_uint4(0, 0, 0, 0);
-        if (J0 + ch * 8 - kb >= T) v = *reinterpret_cast<const uint4*>(W + (size_t)(kb + row) * ldw + J0 + ch * 8);
+        if (!MASKED || J0 + ch * 8 - kb >= T)
+            v = *reinterpret_cast<const uint4*>(W + (size_t)(kb + row) * ldw + J0 + ch * 8);
         *reinterpret_cast<uint4*>(Bs + row * kMacro + ch * 8) = v;
     }
     __syncthreads();
@@ -295,11 +304,11 @@ __device__ __forceinline__ void indep_tile_u16(uint16_t* __restrict__ W, size_t 
         }
     }
 
-    if (J0 + tx * 8 - kb < T) return;  // these columns belong to the pivot block
+    if (MASKED && J0 + tx * 8 - kb < T) return;  // these columns belong to the pivot block
     uint16_t* op = W + (size_t)(I0 + ty) * ldw + J0 + tx * 8;
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
-        if (I0 + ty + 16 * r - kb < T) continue;  // pivot rows are final already
+        if (MASKED && I0 + ty + 16 * r - kb < T) continue;  // pivot rows are final already
         uint4 v;
         v.x = as_u32(acc[r][0]);
         v.y = as_u32(acc[r][1]);
@@ -307,6 +316,16 @@ __device__ __forceinline__ void indep_tile_u16(uint16_t* __restrict__ W, size_t 
         v.w = as_u32(acc[r][3]);
         *reinterpret_cast<uint4*>(op + (size_t)(16 * r) * ldw) = v;
     }
+}
+
+template <uint32_t T, bool F16 = false>
+__device__ __forceinline__ void indep_tile_u16(uint16_t* __restrict__ W, size_t ldw, uint32_t kb, uint32_t I0,
+                                               uint32_t J0, uint16_t* __restrict__ lds) {
+    const bool clean = (I0 + kMacro <= kb || I0 >= kb + T) && (J0 + kMacro <= kb || J0 >= kb + T);
+    if (clean)
+        indep_tile_u16_impl<T, F16, false>(W, ldw, kb, I0, J0, lds);
+    else
+        indep_tile_u16_impl<T, F16, true>(W, ldw, kb, I0, J0, lds);
 }
 
 // Independent tiles, 32-bit.  Lane (ty, tx) owns rows ty + 16 r (r < 8) and columns
