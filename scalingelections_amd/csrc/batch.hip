@@ -53,17 +53,18 @@ struct ElectionMeta {
 };
 
 // ------------------------------------------------------------------------- counting
-// item.x = election, item.y = I, item.z = J, item.w = mode (0: small voter-parallel, 1: tile)
+// item.x = election, item.y = first row I0, item.z = first column J0 (candidates),
+// item.w = mode | rows << 8 | columns << 16; mode 1: full 128 x 128 tile, mode 0:
+// voter-parallel sub-block (small elections and the partial tiles at the edge).
 template <uint32_t K>
 __device__ __forceinline__ void batch_count_item(const int32_t* pos, uint32_t* d, uint32_t C, uint32_t m,
                                                  const uint4 it, uint32_t (*PI)[kCT], uint32_t (*PJ)[kCT]) {
-    if (it.w == 1) {  // full 128 x 128 tile
-        count_tile<K>(pos, 0, m, C, C, d, C, it.y, it.z, PI, PJ, &PI[0][0], false);
+    if ((it.w & 0xFFu) == 1) {
+        count_tile<K>(pos, 0, m, C, C, d, C, it.y / kCT, it.z / kCT, PI, PJ, &PI[0][0], false);
         return;
     }
-    // small election or edge tile: voter-parallel sub-block
-    const uint32_t I0 = it.y * kCT, J0 = it.z * kCT;
-    count_block<K>(pos, 0, m, C, I0, J0, min(kCT, C - I0), min(kCT, C - J0), d, C, PI, PJ, &PI[0][0], false);
+    const uint32_t ri = (it.w >> 8) & 0xFFu, rj = (it.w >> 16) & 0xFFu;
+    count_block<K>(pos, 0, m, C, it.y, it.z, ri, rj, d, C, PI, PJ, &PI[0][0], false);
 }
 
 __global__ __launch_bounds__(256) void batch_count_kernel(const int32_t* __restrict__ pos_all,
@@ -96,15 +97,21 @@ struct BatchLds {
     } u;
 };
 
-template <typename E, uint32_t T>
+template <typename E, uint32_t T, bool F16>
 __device__ __forceinline__ void indep_tile(E* W, size_t ldw, uint32_t kb, uint32_t I0, uint32_t J0, E* lds) {
     if constexpr (sizeof(E) == 2)
-        indep_tile_u16<T>(W, ldw, kb, I0, J0, lds);
+        indep_tile_u16<T, F16>(W, ldw, kb, I0, J0, lds);
     else
         indep_tile_u32<T>(W, ldw, kb, I0, J0, lds);
 }
 
-template <typename E, uint32_t T>
+// Offset of the f16-ordered batch closure: nonzero p0 values v <= m <= 30720 are stored as
+// v + 0x3FF, positive f16 normals in [0x0400, 0x7BFF] whose unsigned order is their value
+// order; the pivot and panel code compares them as integers, the tiles as f16.
+constexpr uint32_t kBatchF16Bias = 0x3FFu;
+constexpr uint32_t kBatchF16MaxVoters = 0x7C00u - 0x0400u;
+
+template <typename E, uint32_t T, bool F16>
 __global__ __launch_bounds__(256) void batch_closure_kernel(const uint32_t* __restrict__ d_all,
                                                             E* __restrict__ W_all, uint32_t* __restrict__ p_all,
                                                             uint32_t* __restrict__ rank_all,
@@ -120,12 +127,13 @@ __global__ __launch_bounds__(256) void batch_closure_kernel(const uint32_t* __re
     if (C == 0) return;
 
     // p0 (scaling_elections.py:103-112) into the padded work matrix
+    constexpr uint32_t bias = F16 ? kBatchF16Bias : 0u;
     for (uint32_t q = t; q < Cp * Cp; q += kThreads) {
         const uint32_t i = q / Cp, j = q % Cp;
         uint32_t v = 0;
         if (i < C && j < C && i != j) {
             const uint32_t x = d[(size_t)i * C + j];
-            if (x > d[(size_t)j * C + i]) v = x;
+            if (x > d[(size_t)j * C + i]) v = x + bias;
         }
         W[q] = (E)v;
     }
@@ -151,7 +159,7 @@ __global__ __launch_bounds__(256) void batch_closure_kernel(const uint32_t* __re
         __syncthreads();
         if (Cp > T) {
             for (uint32_t I0 = 0; I0 < Cp; I0 += kMacro)
-                for (uint32_t J0 = 0; J0 < Cp; J0 += kMacro) indep_tile<E, T>(W, Cp, kb, I0, J0, L.u.indep);
+                for (uint32_t J0 = 0; J0 < Cp; J0 += kMacro) indep_tile<E, T, F16>(W, Cp, kb, I0, J0, L.u.indep);
             __syncthreads();
         }
     }
@@ -159,7 +167,8 @@ __global__ __launch_bounds__(256) void batch_closure_kernel(const uint32_t* __re
     // output with zero diagonal, then wins (:287-290) and the stable ranking (:292)
     for (uint32_t q = t; q < C * C; q += kThreads) {
         const uint32_t i = q / C, j = q % C;
-        p[q] = i == j ? 0u : (uint32_t)W[(size_t)i * Cp + j];
+        const uint32_t w = (uint32_t)W[(size_t)i * Cp + j];
+        p[q] = (i == j || w == 0) ? 0u : w - bias;
     }
     for (uint32_t i = t; i < C; i += kThreads) {
         uint32_t w = 0;
@@ -265,21 +274,27 @@ int batch_device_impl(DeviceContext* ctx, uint32_t E, const uint32_t* Cs, const 
             const uint32_t C = Cs[e];
             if (!C) continue;
             // items: full 128 x 128 tiles (mode 1) and voter-parallel sub-blocks (mode 0) for
-            // small elections and the partial tiles at the edge
+            // small elections and the partial tiles at the edge.  (Cutting full diagonal
+            // tiles into two 64-wide upper triangles and a 64 x 64 square, to skip their
+            // lower half, measured slower: the voter-parallel pieces cost more than the half
+            // tile they save.)
+            const uint32_t K = meta[e].K;
+            const double words = (double)((ms[e] + K - 1) / K);
+            auto add = [&](uint32_t I0, uint32_t J0, uint32_t ri, uint32_t rj, bool full) {
+                items.push_back(make_uint4(e, I0, J0, (full ? 1u : 0u) | (ri << 8) | (rj << 16)));
+                // per-lane instructions: staging (K loads + packing per word) and 128 per
+                // word shared by the G word groups of a voter-parallel item
+                const uint32_t nbi = (ri + 7) / 8, nbj = (rj + 7) / 8;
+                const uint32_t blocks = I0 == J0 ? nbi * (nbi + 1) / 2 : nbi * nbj;
+                const double G = full ? 1.0 : (double)(kCThreads / blocks);
+                cost.push_back(words * (K + 4) + words * 128.0 / G + 64.0);
+            };
             const uint32_t nb = (C + kCT - 1) / kCT;
             for (uint32_t I = 0; I < nb; ++I)
                 for (uint32_t J = I; J < nb; ++J) {
-                    const uint32_t ri = std::min(kCT, C - I * kCT), rj = std::min(kCT, C - J * kCT);
-                    const bool full = ri == kCT && rj == kCT;
-                    const uint32_t nbi = (ri + 7) / 8, nbj = (rj + 7) / 8;
-                    const uint32_t blocks = I == J ? nbi * (nbi + 1) / 2 : nbi * nbj;
-                    items.push_back(make_uint4(e, I, J, full ? 1 : 0));
-                    // per-lane instructions: staging (K loads + packing per word) and 128
-                    // per word shared by the G word groups of a voter-parallel item
-                    const uint32_t K = meta[e].K;
-                    const double words = (double)((ms[e] + K - 1) / K);
-                    const double G = full ? 1.0 : (double)(kCThreads / blocks);
-                    cost.push_back(words * (K + 4) + words * 128.0 / G + 64.0);
+                    const uint32_t I0 = I * kCT, J0 = J * kCT;
+                    const uint32_t ri = std::min(kCT, C - I0), rj = std::min(kCT, C - J0);
+                    add(I0, J0, ri, rj, ri == kCT && rj == kCT);
                 }
         }
         std::vector<uint32_t> idx(items.size());
@@ -315,11 +330,14 @@ int batch_device_impl(DeviceContext* ctx, uint32_t E, const uint32_t* Cs, const 
         SE_CHECK_LAUNCH("batch counter launch failed");
         if (timing) (void)hipEventRecord(ev[1], s);
         if (E) {
-            if (use16)
-                batch_closure_kernel<uint16_t, 64>
+            if (use16 && max_m <= kBatchF16MaxVoters)
+                batch_closure_kernel<uint16_t, 64, true>
+                    <<<E, kThreads, 0, s>>>(dbuf, static_cast<uint16_t*>(wbuf), paths, ranking, dmeta, dorder);
+            else if (use16)
+                batch_closure_kernel<uint16_t, 64, false>
                     <<<E, kThreads, 0, s>>>(dbuf, static_cast<uint16_t*>(wbuf), paths, ranking, dmeta, dorder);
             else
-                batch_closure_kernel<uint32_t, 32>
+                batch_closure_kernel<uint32_t, 32, false>
                     <<<E, kThreads, 0, s>>>(dbuf, static_cast<uint32_t*>(wbuf), paths, ranking, dmeta, dorder);
         }
         SE_CHECK_LAUNCH("batch closure launch failed");

@@ -69,6 +69,17 @@ def test_batch_many_voters_every_packing(gpu_lib, oracle_mod):
     _check(oracle_mod, batch.make_workload(C, m, np.array([0.9, 0.95, 0.97, 0.6]), seed=14), "voters")
 
 
+def test_batch_16bit_integer_closure(gpu_lib, oracle_mod):
+    """More than 30720 voters but at most 65535: the batch closure runs 16-bit integer
+    tiles instead of the f16-ordered ones (whose offset needs values up to 30720)."""
+    from scalingelections_amd import batch
+
+    C = np.array([100, 200, 129, 17], dtype=np.uint32)
+    m = np.array([40000, 33000, 31000, 30721], dtype=np.uint32)
+    print("batch 16-bit integer", C.tolist(), m.tolist(), flush=True)
+    _check(oracle_mod, batch.make_workload(C, m, np.array([0.8, 0.99, 0.7, 0.95]), seed=15), "u16")
+
+
 def test_batch_mixed_sample(gpu_lib, oracle_mod):
     """200 elections from the mixed-size distribution (smaller voter counts); all compared
     batched vs per-launch, 40 against the oracle."""
