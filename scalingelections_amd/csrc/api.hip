@@ -127,6 +127,7 @@ static void free_context_scratch(DeviceContext* ctx) {
         ctx->ptr[i] = nullptr;
         ctx->cap[i] = 0;
     }
+    free_pinned(ctx);
 }
 
 }  // namespace se
@@ -206,21 +207,19 @@ int se_closure(const uint32_t* prefs, uint32_t n, size_t row_stride, uint32_t* p
     uint32_t* d = static_cast<uint32_t*>(scratch(ctx, kSlotPrefs, bytes));
     if (!d) return SE_ERR_ALLOC;
     hipStream_t s = ctx->stream;
-    if (!hip_ok(hipMemcpy2DAsync(d, n * sizeof(uint32_t), prefs, row_stride * sizeof(uint32_t),
-                                 n * sizeof(uint32_t), n, hipMemcpyHostToDevice, s),
-                "Failed to copy data from host to device"))
-        return SE_ERR_COPY;
+    int rc = upload_rows(ctx, s, d, prefs, row_stride * sizeof(uint32_t), n, n * sizeof(uint32_t));
+    if (rc != SE_OK) {
+        (void)hipStreamSynchronize(s);
+        return rc;
+    }
     // the staged input is dead after p0 init, so the output reuses its buffer
-    int rc = closure_device_impl(ctx, d, n, n, d, n, flags, s, timing);
+    rc = closure_device_impl(ctx, d, n, n, d, n, flags, s, timing);
     if (rc != SE_OK) {
         (void)hipStreamSynchronize(s);
         return rc;
     }
     if (!hip_ok(hipStreamSynchronize(s), "CUDA operations did not complete successfully")) return SE_ERR_DEVICE;
-    if (!hip_ok(hipMemcpyAsync(paths, d, bytes, hipMemcpyDeviceToHost, s), "Failed to copy data from device to host"))
-        return SE_ERR_COPY;
-    if (!hip_ok(hipStreamSynchronize(s), "CUDA transfers did not complete successfully")) return SE_ERR_COPY;
-    return SE_OK;
+    return download_rows(ctx, s, paths, n * sizeof(uint32_t), d, n, n * sizeof(uint32_t));
 }
 
 int se_closure_device(const uint32_t* prefs_dev, uint32_t n, size_t row_stride, uint32_t* paths_dev,
@@ -299,19 +298,19 @@ int se_count(const int32_t* positions, uint32_t m, uint32_t n, uint32_t* prefs, 
     if (!d) return SE_ERR_ALLOC;
     uint32_t* flag = static_cast<uint32_t*>(scratch(ctx, kSlotCountFlag, 256));
     if (!flag) return SE_ERR_ALLOC;
-    if (m > 0 && !hip_ok(hipMemcpyAsync(pos, positions, pbytes, hipMemcpyHostToDevice, s),
-                         "Failed to copy data from host to device"))
-        return SE_ERR_COPY;
-    int rc = clear_flag(flag, s);
+    int rc = m > 0 ? upload_rows(ctx, s, pos, positions, n * sizeof(int32_t), m, n * sizeof(int32_t)) : SE_OK;
+    if (rc == SE_OK) rc = clear_flag(flag, s);
     if (rc == SE_OK) rc = count_device_impl(ctx, pos, m, n, d, n, s, nullptr, flag);
-    if (rc != SE_OK) return rc;
+    if (rc != SE_OK) {
+        (void)hipStreamSynchronize(s);
+        return rc;
+    }
     uint32_t bad = 0;
-    if (!hip_ok(hipMemcpy2DAsync(prefs, prefs_stride * sizeof(uint32_t), d, n * sizeof(uint32_t),
-                                 n * sizeof(uint32_t), n, hipMemcpyDeviceToHost, s),
-                "Failed to copy data from device to host") ||
-        !hip_ok(hipMemcpyAsync(&bad, flag, sizeof(bad), hipMemcpyDeviceToHost, s),
+    if (!hip_ok(hipMemcpyAsync(&bad, flag, sizeof(bad), hipMemcpyDeviceToHost, s),
                 "Failed to copy data from device to host"))
         return SE_ERR_COPY;
+    rc = download_rows(ctx, s, prefs, prefs_stride * sizeof(uint32_t), d, n, n * sizeof(uint32_t));
+    if (rc != SE_OK) return rc;
     if (!hip_ok(hipStreamSynchronize(s), "CUDA transfers did not complete successfully")) return SE_ERR_COPY;
     if (bad) {
         set_error("positions out of range: every ballot row must be a permutation of 0..n-1");

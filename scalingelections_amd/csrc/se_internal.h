@@ -46,7 +46,18 @@ struct DeviceContext {
     std::mutex lock;
     void* ptr[kSlotCount] = {};
     size_t cap[kSlotCount] = {};
+    void* pinned[2] = {nullptr, nullptr};          // pinned staging chunks (transfer.hip)
+    hipEvent_t pinned_ev[2] = {nullptr, nullptr};  // last DMA of each chunk
 };
+
+// Host <-> device matrix copies through the context's pinned staging chunks
+// (transfer.hip).  upload_rows returns once the last chunk is queued on s; download_rows
+// returns after the data is in host memory.
+int upload_rows(DeviceContext* ctx, hipStream_t s, void* dev, const void* host, size_t host_pitch, size_t rows,
+                size_t row_bytes);
+int download_rows(DeviceContext* ctx, hipStream_t s, void* host, size_t host_pitch, const void* dev, size_t rows,
+                  size_t row_bytes);
+void free_pinned(DeviceContext* ctx);
 
 // Context of the current HIP device (created on first use).  Returns nullptr and sets
 // the error when no device is usable.

@@ -1,0 +1,210 @@
+// transfer.hip -- host <-> device copies of n x n matrices through pinned staging.
+//
+// The host entry points receive and return pageable caller buffers (the drop-in
+// compute_strongest_paths returns a fresh numpy array).  A plain hipMemcpy from pageable
+// memory moves about 10 GB/s; here each matrix goes through two pinned chunks of up to
+// 64 MiB: host worker threads copy rows between the caller's buffer and one chunk while
+// the DMA engine moves the other, so the copy runs at the DMA rate.  The chunks are
+// allocated once per device context (hipHostMalloc) and reused.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <condition_variable>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "se_internal.h"
+
+namespace se {
+
+namespace {
+
+constexpr size_t kChunkBytes = 64u << 20;
+
+// Fixed pool of host threads for row copies.  run(count, fn) calls fn(i) for i < count
+// from the workers and the calling thread and returns when all are done.
+class RowPool {
+  public:
+    static RowPool& get() {
+        static RowPool pool;
+        return pool;
+    }
+
+    void run(size_t count, const std::function<void(size_t)>& fn) {
+        std::lock_guard<std::mutex> serial(call_mutex_);  // one job at a time
+        if (count == 0) return;
+        if (workers_.empty() || count < 4) {
+            for (size_t i = 0; i < count; ++i) fn(i);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> g(m_);
+            job_ = &fn;
+            count_ = count;
+            next_.store(0);
+            active_ = workers_.size();
+            ++generation_;
+        }
+        cv_.notify_all();
+        drain();
+        std::unique_lock<std::mutex> g(m_);
+        done_cv_.wait(g, [&] { return active_ == 0; });
+        job_ = nullptr;
+    }
+
+  private:
+    RowPool() {
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        const unsigned n = std::min(15u, hw > 1 ? hw - 1 : 0u);
+        for (unsigned i = 0; i < n; ++i) workers_.emplace_back([this] { loop(); });
+    }
+    ~RowPool() {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            stop_ = true;
+            ++generation_;
+        }
+        cv_.notify_all();
+        for (auto& w : workers_) w.join();
+    }
+
+    void drain() {
+        for (;;) {
+            const size_t i = next_.fetch_add(1);
+            if (i >= count_) break;
+            (*job_)(i);
+        }
+    }
+
+    void loop() {
+        uint64_t seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> g(m_);
+                cv_.wait(g, [&] { return stop_ || generation_ != seen; });
+                if (stop_) return;
+                seen = generation_;
+            }
+            drain();
+            std::lock_guard<std::mutex> g(m_);
+            if (--active_ == 0) done_cv_.notify_all();
+        }
+    }
+
+    std::vector<std::thread> workers_;
+    std::mutex call_mutex_, m_;
+    std::condition_variable cv_, done_cv_;
+    const std::function<void(size_t)>* job_ = nullptr;
+    size_t count_ = 0;
+    std::atomic<size_t> next_{0};
+    size_t active_ = 0;
+    uint64_t generation_ = 0;
+    bool stop_ = false;
+};
+
+// Copies `rows` rows of `row_bytes` between strided host memory and a packed buffer, in
+// parallel blocks of rows.
+void copy_rows(char* dst, size_t dst_pitch, const char* src, size_t src_pitch, size_t rows, size_t row_bytes) {
+    const size_t per = std::max<size_t>(1, (1u << 20) / std::max<size_t>(1, row_bytes));  // about 1 MiB per task
+    const size_t tasks = (rows + per - 1) / per;
+    RowPool::get().run(tasks, [&](size_t t) {
+        const size_t r0 = t * per, r1 = std::min(rows, r0 + per);
+        if (dst_pitch == row_bytes && src_pitch == row_bytes) {
+            std::memcpy(dst + r0 * dst_pitch, src + r0 * src_pitch, (r1 - r0) * row_bytes);
+        } else {
+            for (size_t r = r0; r < r1; ++r) std::memcpy(dst + r * dst_pitch, src + r * src_pitch, row_bytes);
+        }
+    });
+}
+
+bool ensure_pinned(DeviceContext* ctx) {
+    if (ctx->pinned[0]) return true;
+    for (int b = 0; b < 2; ++b) {
+        if (!hip_ok(hipHostMalloc(&ctx->pinned[b], kChunkBytes, hipHostMallocDefault),
+                    "Failed to allocate pinned host memory") ||
+            !hip_ok(hipEventCreateWithFlags(&ctx->pinned_ev[b], hipEventDisableTiming), "Failed to create events"))
+            return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+// host (rows x row_bytes, pitch host_pitch) -> device (packed, pitch row_bytes)
+int upload_rows(DeviceContext* ctx, hipStream_t s, void* dev, const void* host, size_t host_pitch, size_t rows,
+                size_t row_bytes) {
+    if (rows == 0 || row_bytes == 0) return SE_OK;
+    if (row_bytes > kChunkBytes || !ensure_pinned(ctx)) {  // fall back to one plain copy
+        return hip_ok(hipMemcpy2DAsync(dev, row_bytes, host, host_pitch, row_bytes, rows, hipMemcpyHostToDevice, s),
+                      "Failed to copy data from host to device")
+                   ? SE_OK
+                   : SE_ERR_COPY;
+    }
+    const size_t per = kChunkBytes / row_bytes;
+    bool used[2] = {false, false};
+    for (size_t r0 = 0, c = 0; r0 < rows; r0 += per, ++c) {
+        const size_t nr = std::min(per, rows - r0);
+        const int b = (int)(c & 1);
+        if (used[b] && !hip_ok(hipEventSynchronize(ctx->pinned_ev[b]), "Failed to copy data from host to device"))
+            return SE_ERR_COPY;
+        copy_rows(static_cast<char*>(ctx->pinned[b]), row_bytes, static_cast<const char*>(host) + r0 * host_pitch,
+                  host_pitch, nr, row_bytes);
+        if (!hip_ok(hipMemcpyAsync(static_cast<char*>(dev) + r0 * row_bytes, ctx->pinned[b], nr * row_bytes,
+                                   hipMemcpyHostToDevice, s),
+                    "Failed to copy data from host to device") ||
+            !hip_ok(hipEventRecord(ctx->pinned_ev[b], s), "Failed to copy data from host to device"))
+            return SE_ERR_COPY;
+        used[b] = true;
+    }
+    return SE_OK;
+}
+
+// device (packed, pitch row_bytes) -> host (pitch host_pitch); returns after the data landed
+int download_rows(DeviceContext* ctx, hipStream_t s, void* host, size_t host_pitch, const void* dev, size_t rows,
+                  size_t row_bytes) {
+    if (rows == 0 || row_bytes == 0) return SE_OK;
+    if (row_bytes > kChunkBytes || !ensure_pinned(ctx)) {
+        if (!hip_ok(hipMemcpy2DAsync(host, host_pitch, dev, row_bytes, row_bytes, rows, hipMemcpyDeviceToHost, s),
+                    "Failed to copy data from device to host"))
+            return SE_ERR_COPY;
+        return hip_ok(hipStreamSynchronize(s), "CUDA transfers did not complete successfully") ? SE_OK : SE_ERR_COPY;
+    }
+    const size_t per = kChunkBytes / row_bytes;
+    const size_t chunks = (rows + per - 1) / per;
+    auto issue = [&](size_t c) {
+        const size_t r0 = c * per, nr = std::min(per, rows - r0);
+        const int b = (int)(c & 1);
+        return hip_ok(hipMemcpyAsync(ctx->pinned[b], static_cast<const char*>(dev) + r0 * row_bytes, nr * row_bytes,
+                                     hipMemcpyDeviceToHost, s),
+                      "Failed to copy data from device to host") &&
+               hip_ok(hipEventRecord(ctx->pinned_ev[b], s), "Failed to copy data from device to host");
+    };
+    if (!issue(0)) return SE_ERR_COPY;
+    for (size_t c = 0; c < chunks; ++c) {
+        // the other buffer's host copy finished in the previous iteration: refill it
+        if (c + 1 < chunks && !issue(c + 1)) return SE_ERR_COPY;
+        const int b = (int)(c & 1);
+        if (!hip_ok(hipEventSynchronize(ctx->pinned_ev[b]), "CUDA transfers did not complete successfully"))
+            return SE_ERR_COPY;
+        const size_t r0 = c * per, nr = std::min(per, rows - r0);
+        copy_rows(static_cast<char*>(host) + r0 * host_pitch, host_pitch, static_cast<const char*>(ctx->pinned[b]),
+                  row_bytes, nr, row_bytes);
+    }
+    return SE_OK;
+}
+
+void free_pinned(DeviceContext* ctx) {
+    for (int b = 0; b < 2; ++b) {
+        if (ctx->pinned[b]) (void)hipHostFree(ctx->pinned[b]);
+        if (ctx->pinned_ev[b]) (void)hipEventDestroy(ctx->pinned_ev[b]);
+        ctx->pinned[b] = nullptr;
+        ctx->pinned_ev[b] = nullptr;
+    }
+}
+
+}  // namespace se
