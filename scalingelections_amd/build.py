@@ -27,7 +27,9 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
-HOST_ONLY = {"cpu_closure.cpp": ["-march=x86-64-v3"], "synth.cpp": ["-march=x86-64-v3"]}
+HOST_ONLY = {"cpu_closure.cpp": ["-march=x86-64-v3"], "synth.cpp": ["-march=x86-64-v3"],
+             # host-side row narrowing/widening loops of the staged copies (AVX2)
+             "transfer.hip": ["-Xarch_host", "-march=x86-64-v3"]}
 
 
 def _hipcc() -> str:

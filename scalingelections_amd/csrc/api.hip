@@ -19,6 +19,8 @@ namespace se {
 // Implemented in closure.hip / count.hip / rank.hip / cpu_closure.cpp.
 int closure_device_impl(DeviceContext* ctx, const uint32_t* d, uint32_t n, size_t ld, uint32_t* out,
                         size_t ldo, uint32_t flags, hipStream_t s, se_timing* timing);
+int closure_device_ex(DeviceContext* ctx, const void* d_any, bool d16, uint32_t n, size_t ld, void* out_any,
+                      size_t ldo, bool* out16, uint32_t flags, hipStream_t s, se_timing* timing);
 int count_device_impl(DeviceContext* ctx, const int32_t* pos, uint32_t m, uint32_t n, uint32_t* d,
                       size_t ldd, hipStream_t s, double* kernel_ms, uint32_t* flag);
 int clear_flag(uint32_t* flag, hipStream_t s);
@@ -207,18 +209,28 @@ int se_closure(const uint32_t* prefs, uint32_t n, size_t row_stride, uint32_t* p
     uint32_t* d = static_cast<uint32_t*>(scratch(ctx, kSlotPrefs, bytes));
     if (!d) return SE_ERR_ALLOC;
     hipStream_t s = ctx->stream;
-    int rc = upload_rows(ctx, s, d, prefs, row_stride * sizeof(uint32_t), n, n * sizeof(uint32_t));
+    // counts up to 65535 cross PCIe as 16-bit values (narrowed and widened on the host
+    // while staged), larger ones as 32-bit; a forced 32-bit closure keeps 32-bit transit
+    bool d16 = false;
+    int rc = SE_NARROW_FAILED;
+    if (!(flags & SE_FLAG_FORCE_U32)) {
+        rc = upload_rows_narrow16(ctx, s, reinterpret_cast<uint16_t*>(d), prefs, row_stride * sizeof(uint32_t), n, n);
+        d16 = rc == SE_OK;
+    }
+    if (rc == SE_NARROW_FAILED) rc = upload_rows(ctx, s, d, prefs, row_stride * sizeof(uint32_t), n, n * sizeof(uint32_t));
     if (rc != SE_OK) {
         (void)hipStreamSynchronize(s);
         return rc;
     }
     // the staged input is dead after p0 init, so the output reuses its buffer
-    rc = closure_device_impl(ctx, d, n, n, d, n, flags, s, timing);
+    bool out16 = false;
+    rc = closure_device_ex(ctx, d, d16, n, n, d, n, &out16, flags, s, timing);
     if (rc != SE_OK) {
         (void)hipStreamSynchronize(s);
         return rc;
     }
     if (!hip_ok(hipStreamSynchronize(s), "CUDA operations did not complete successfully")) return SE_ERR_DEVICE;
+    if (out16) return download_rows_widen16(ctx, s, paths, n * sizeof(uint32_t), reinterpret_cast<uint16_t*>(d), n, n);
     return download_rows(ctx, s, paths, n * sizeof(uint32_t), d, n, n * sizeof(uint32_t));
 }
 

@@ -54,16 +54,17 @@ __device__ __forceinline__ void upper_pair(uint32_t t, uint32_t nt, uint32_t* I,
 
 // Loads d block (I, J) into registers (row a = ty + 4 q) and block (J, I) transposed into
 // tr (tr[b][a] = d[J0 + b][I0 + a]); zeros outside n x n.
-__device__ __forceinline__ void load_pair(const uint32_t* __restrict__ d, uint32_t n, size_t ld, uint32_t I0,
+template <typename D>
+__device__ __forceinline__ void load_pair(const D* __restrict__ d, uint32_t n, size_t ld, uint32_t I0,
                                           uint32_t J0, uint32_t (*tr)[kInitTile + 1], uint32_t* v) {
     const uint32_t tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
 #pragma unroll
     for (uint32_t q = 0; q < kInitTile / 4; ++q) {
         const uint32_t a = ty + 4 * q;
         const uint32_t i = I0 + a, j = J0 + tx;
-        v[q] = (i < n && j < n) ? d[(size_t)i * ld + j] : 0u;
+        v[q] = (i < n && j < n) ? (uint32_t)d[(size_t)i * ld + j] : 0u;
         const uint32_t r = J0 + a, c = I0 + tx;
-        tr[tx][a] = (r < n && c < n) ? d[(size_t)r * ld + c] : 0u;  // tr[c - I0][r - J0]
+        tr[tx][a] = (r < n && c < n) ? (uint32_t)d[(size_t)r * ld + c] : 0u;  // tr[c - I0][r - J0]
     }
 }
 
@@ -71,7 +72,9 @@ __device__ __forceinline__ void load_pair(const uint32_t* __restrict__ d, uint32
 // value (atomicMin, stays 0xFFFFFFFF when p0 is all zero).  Decides the 16-bit eligibility
 // and whether the values fit the f16-ordered range.  A pair (i, j), i != j, contributes
 // max(d[i][j], d[j][i]) when the two differ (that larger count is a nonzero p0 entry).
-__global__ __launch_bounds__(256) void p0_max_kernel(const uint32_t* __restrict__ d, uint32_t n,
+// D: element type of d (uint32_t, or uint16_t when the host entry narrowed it in transit).
+template <typename D>
+__global__ __launch_bounds__(256) void p0_max_kernel(const D* __restrict__ d, uint32_t n,
                                                      size_t ld, uint32_t nt, uint32_t* __restrict__ out) {
     __shared__ uint32_t tr[kInitTile][kInitTile + 1];
     __shared__ uint32_t red[kThreads / 64], redmin[kThreads / 64];
@@ -118,8 +121,8 @@ __global__ __launch_bounds__(256) void p0_max_kernel(const uint32_t* __restrict_
 // orientations of each tile pair; nonzero values are stored as v + bias (bias = 0, or the
 // f16-ordered offset 0x0400 - min).  Block (J, I) is assembled transposed in LDS, then
 // written row-major.
-template <typename E>
-__global__ __launch_bounds__(256) void p0_write_kernel(const uint32_t* __restrict__ d, uint32_t n,
+template <typename E, typename D>
+__global__ __launch_bounds__(256) void p0_write_kernel(const D* __restrict__ d, uint32_t n,
                                                        size_t ld, E* __restrict__ W, size_t ldw, uint32_t nt,
                                                        uint32_t bias) {
     __shared__ uint32_t tr[kInitTile][kInitTile + 1];
@@ -282,15 +285,15 @@ __global__ __launch_bounds__(256) void panel_indep_u16_kernel(uint16_t* __restri
 // ---------------------------------------------------------------------------------------
 // Output: out[i][j] = W[i][j] widened to u32, 0 on the diagonal, padding dropped.
 
-template <typename E>
+template <typename E, typename O>
 __global__ __launch_bounds__(256) void output_kernel(const E* __restrict__ W, size_t ldw, uint32_t n,
-                                                     uint32_t* __restrict__ out, size_t ldo, uint32_t bias) {
+                                                     O* __restrict__ out, size_t ldo, uint32_t bias) {
     const uint32_t i = blockIdx.x;
     const E* src = W + (size_t)i * ldw;
-    uint32_t* dst = out + (size_t)i * ldo;
+    O* dst = out + (size_t)i * ldo;
     for (uint32_t j = threadIdx.x; j < n; j += kThreads) {
         const uint32_t w = (uint32_t)src[j];
-        dst[j] = (i == j || w == 0) ? 0u : w - bias;
+        dst[j] = (O)((i == j || w == 0) ? 0u : w - bias);
     }
 }
 
@@ -487,13 +490,19 @@ int run_rounds16(uint16_t* W, size_t ldw, uint32_t n, uint32_t N, uint32_t* pivo
 
 }  // namespace
 
-// Device pipeline: d (device, n x n u32, row stride ld) -> out (device, n x n u32, ldo).
-int closure_device_impl(DeviceContext* ctx, const uint32_t* d, uint32_t n, size_t ld, uint32_t* out,
-                        size_t ldo, uint32_t flags, hipStream_t s, se_timing* timing) {
+// Device pipeline: d (device, n x n, row stride ld, u32 or, with d16, u16) -> out
+// (device, n x n, ldo).  out is u32, except that with out16 != nullptr and a 16-bit
+// closure it receives u16 values (every strongest path is then at most 65535) and
+// *out16 is set; the host entry points use both to halve their PCIe traffic.
+int closure_device_ex(DeviceContext* ctx, const void* d_any, bool d16, uint32_t n, size_t ld, void* out_any,
+                      size_t ldo, bool* out16, uint32_t flags, hipStream_t s, se_timing* timing) {
+    if (out16) *out16 = false;
     if (n == 0) {
         if (timing) *timing = se_timing{};
         return SE_OK;
     }
+    const uint32_t* d32 = static_cast<const uint32_t*>(d_any);
+    const uint16_t* d16p = static_cast<const uint16_t*>(d_any);
     const uint32_t N = (uint32_t)round_up(n, kMacro);
     const size_t ldw = N;
     const bool want_timing = timing != nullptr;
@@ -520,7 +529,10 @@ int closure_device_impl(DeviceContext* ctx, const uint32_t* d, uint32_t n, size_
         tm.mark(kKindInit);
         init_range_kernel<<<1, 64, 0, s>>>(dmax);
         const uint32_t ntn = (n + kInitTile - 1) / kInitTile;
-        p0_max_kernel<<<init_grid(ntn), kThreads, 0, s>>>(d, n, ld, ntn, dmax);
+        if (d16)
+            p0_max_kernel<uint16_t><<<init_grid(ntn), kThreads, 0, s>>>(d16p, n, ld, ntn, dmax);
+        else
+            p0_max_kernel<uint32_t><<<init_grid(ntn), kThreads, 0, s>>>(d32, n, ld, ntn, dmax);
         tm.mark(kKindInit);
         SE_CHECK_LAUNCH("p0 init launch failed");
         if (!hip_ok(hipMemcpyAsync(range, dmax, sizeof(range), hipMemcpyDeviceToHost, s),
@@ -545,11 +557,18 @@ int closure_device_impl(DeviceContext* ctx, const uint32_t* d, uint32_t n, size_
 
     const uint32_t ntN = N / kInitTile;
     tm.mark(kKindInit);
-    if (use16)
-        p0_write_kernel<uint16_t><<<init_grid(ntN), kThreads, 0, s>>>(d, n, ld, static_cast<uint16_t*>(W), ldw, ntN,
-                                                                     bias);
+    if (use16 && d16)
+        p0_write_kernel<uint16_t, uint16_t>
+            <<<init_grid(ntN), kThreads, 0, s>>>(d16p, n, ld, static_cast<uint16_t*>(W), ldw, ntN, bias);
+    else if (use16)
+        p0_write_kernel<uint16_t, uint32_t>
+            <<<init_grid(ntN), kThreads, 0, s>>>(d32, n, ld, static_cast<uint16_t*>(W), ldw, ntN, bias);
+    else if (d16)
+        p0_write_kernel<uint32_t, uint16_t>
+            <<<init_grid(ntN), kThreads, 0, s>>>(d16p, n, ld, static_cast<uint32_t*>(W), ldw, ntN, 0u);
     else
-        p0_write_kernel<uint32_t><<<init_grid(ntN), kThreads, 0, s>>>(d, n, ld, static_cast<uint32_t*>(W), ldw, ntN, 0u);
+        p0_write_kernel<uint32_t, uint32_t>
+            <<<init_grid(ntN), kThreads, 0, s>>>(d32, n, ld, static_cast<uint32_t*>(W), ldw, ntN, 0u);
     tm.mark(kKindInit);
     SE_CHECK_LAUNCH("p0 init launch failed");
 
@@ -588,10 +607,17 @@ int closure_device_impl(DeviceContext* ctx, const uint32_t* d, uint32_t n, size_
 
     dim3 go(n);
     tm.mark(kKindOut);
-    if (use16)
-        output_kernel<uint16_t><<<go, kThreads, 0, s>>>(static_cast<uint16_t*>(W), ldw, n, out, ldo, bias);
-    else
-        output_kernel<uint32_t><<<go, kThreads, 0, s>>>(static_cast<uint32_t*>(W), ldw, n, out, ldo, 0u);
+    if (use16 && out16) {
+        output_kernel<uint16_t, uint16_t>
+            <<<go, kThreads, 0, s>>>(static_cast<uint16_t*>(W), ldw, n, static_cast<uint16_t*>(out_any), ldo, bias);
+        *out16 = true;
+    } else if (use16) {
+        output_kernel<uint16_t, uint32_t>
+            <<<go, kThreads, 0, s>>>(static_cast<uint16_t*>(W), ldw, n, static_cast<uint32_t*>(out_any), ldo, bias);
+    } else {
+        output_kernel<uint32_t, uint32_t>
+            <<<go, kThreads, 0, s>>>(static_cast<uint32_t*>(W), ldw, n, static_cast<uint32_t*>(out_any), ldo, 0u);
+    }
     tm.mark(kKindOut);
     SE_CHECK_LAUNCH("output launch failed");
 
@@ -621,6 +647,11 @@ int closure_device_impl(DeviceContext* ctx, const uint32_t* d, uint32_t n, size_
         (void)hipEventDestroy(e1);
     }
     return SE_OK;
+}
+
+int closure_device_impl(DeviceContext* ctx, const uint32_t* d, uint32_t n, size_t ld, uint32_t* out,
+                        size_t ldo, uint32_t flags, hipStream_t s, se_timing* timing) {
+    return closure_device_ex(ctx, d, false, n, ld, out, ldo, nullptr, flags, s, timing);
 }
 
 }  // namespace se

@@ -58,6 +58,13 @@ int upload_rows(DeviceContext* ctx, hipStream_t s, void* dev, const void* host, 
 int download_rows(DeviceContext* ctx, hipStream_t s, void* host, size_t host_pitch, const void* dev, size_t rows,
                   size_t row_bytes);
 void free_pinned(DeviceContext* ctx);
+// 16-bit transit: u32 host matrix narrowed into a u16 device matrix (SE_NARROW_FAILED when
+// a value exceeds 65535), and a u16 device matrix widened into a u32 host matrix.
+constexpr int SE_NARROW_FAILED = 1;
+int upload_rows_narrow16(DeviceContext* ctx, hipStream_t s, uint16_t* dev, const uint32_t* host, size_t host_pitch,
+                         size_t rows, size_t n);
+int download_rows_widen16(DeviceContext* ctx, hipStream_t s, uint32_t* host, size_t host_pitch, const uint16_t* dev,
+                          size_t rows, size_t n);
 
 // Context of the current HIP device (created on first use).  Returns nullptr and sets
 // the error when no device is usable.

@@ -198,6 +198,91 @@ int download_rows(DeviceContext* ctx, hipStream_t s, void* host, size_t host_pit
     return SE_OK;
 }
 
+// host u32 rows -> device u16 rows (n values each).  The values are narrowed while they
+// are staged; SE_NARROW_FAILED when one exceeds 65535 (the device copy is then incomplete
+// and the caller uploads 32-bit values instead).
+int upload_rows_narrow16(DeviceContext* ctx, hipStream_t s, uint16_t* dev, const uint32_t* host, size_t host_pitch,
+                         size_t rows, size_t n) {
+    const size_t row_bytes = n * sizeof(uint16_t);
+    if (rows == 0 || n == 0) return SE_OK;
+    if (row_bytes > kChunkBytes || !ensure_pinned(ctx)) return SE_NARROW_FAILED;
+    const size_t per = kChunkBytes / row_bytes;
+    bool used[2] = {false, false};
+    std::atomic<uint32_t> high{0};
+    for (size_t r0 = 0, c = 0; r0 < rows; r0 += per, ++c) {
+        const size_t nr = std::min(per, rows - r0);
+        const int b = (int)(c & 1);
+        if (used[b] && !hip_ok(hipEventSynchronize(ctx->pinned_ev[b]), "Failed to copy data from host to device"))
+            return SE_ERR_COPY;
+        uint16_t* stage = static_cast<uint16_t*>(ctx->pinned[b]);
+        const size_t rows_per_task = std::max<size_t>(1, (1u << 20) / row_bytes);
+        RowPool::get().run((nr + rows_per_task - 1) / rows_per_task, [&](size_t t) {
+            const size_t a0 = t * rows_per_task, a1 = std::min(nr, a0 + rows_per_task);
+            uint32_t acc = 0;
+            for (size_t r = a0; r < a1; ++r) {
+                const uint32_t* src =
+                    reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(host) + (r0 + r) * host_pitch);
+                uint16_t* dst = stage + r * n;
+                for (size_t j = 0; j < n; ++j) {
+                    acc |= src[j];
+                    dst[j] = (uint16_t)src[j];
+                }
+            }
+            if (acc >> 16) high.fetch_or(1);
+        });
+        if (high.load()) {
+            (void)hipStreamSynchronize(s);
+            return SE_NARROW_FAILED;
+        }
+        if (!hip_ok(hipMemcpyAsync(dev + r0 * n, stage, nr * row_bytes, hipMemcpyHostToDevice, s),
+                    "Failed to copy data from host to device") ||
+            !hip_ok(hipEventRecord(ctx->pinned_ev[b], s), "Failed to copy data from host to device"))
+            return SE_ERR_COPY;
+        used[b] = true;
+    }
+    return SE_OK;
+}
+
+// device u16 rows (n values each) -> host u32 rows (pitch host_pitch), widened while they
+// are unstaged; returns after the data is in host memory.
+int download_rows_widen16(DeviceContext* ctx, hipStream_t s, uint32_t* host, size_t host_pitch, const uint16_t* dev,
+                          size_t rows, size_t n) {
+    const size_t row_bytes = n * sizeof(uint16_t);
+    if (rows == 0 || n == 0) return SE_OK;
+    if (row_bytes > kChunkBytes || !ensure_pinned(ctx)) {
+        set_error("Failed to copy data from device to host");
+        return SE_ERR_COPY;
+    }
+    const size_t per = kChunkBytes / row_bytes;
+    const size_t chunks = (rows + per - 1) / per;
+    auto issue = [&](size_t c) {
+        const size_t r0 = c * per, nr = std::min(per, rows - r0);
+        const int b = (int)(c & 1);
+        return hip_ok(hipMemcpyAsync(ctx->pinned[b], dev + r0 * n, nr * row_bytes, hipMemcpyDeviceToHost, s),
+                      "Failed to copy data from device to host") &&
+               hip_ok(hipEventRecord(ctx->pinned_ev[b], s), "Failed to copy data from device to host");
+    };
+    if (!issue(0)) return SE_ERR_COPY;
+    for (size_t c = 0; c < chunks; ++c) {
+        if (c + 1 < chunks && !issue(c + 1)) return SE_ERR_COPY;
+        const int b = (int)(c & 1);
+        if (!hip_ok(hipEventSynchronize(ctx->pinned_ev[b]), "CUDA transfers did not complete successfully"))
+            return SE_ERR_COPY;
+        const size_t r0 = c * per, nr = std::min(per, rows - r0);
+        const uint16_t* stage = static_cast<const uint16_t*>(ctx->pinned[b]);
+        const size_t rows_per_task = std::max<size_t>(1, (1u << 20) / row_bytes);
+        RowPool::get().run((nr + rows_per_task - 1) / rows_per_task, [&](size_t t) {
+            const size_t a0 = t * rows_per_task, a1 = std::min(nr, a0 + rows_per_task);
+            for (size_t r = a0; r < a1; ++r) {
+                uint32_t* dst = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(host) + (r0 + r) * host_pitch);
+                const uint16_t* src = stage + r * n;
+                for (size_t j = 0; j < n; ++j) dst[j] = src[j];
+            }
+        });
+    }
+    return SE_OK;
+}
+
 void free_pinned(DeviceContext* ctx) {
     for (int b = 0; b < 2; ++b) {
         if (ctx->pinned[b]) (void)hipHostFree(ctx->pinned[b]);
