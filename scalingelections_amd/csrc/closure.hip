@@ -248,6 +248,26 @@ __global__ __launch_bounds__(256) void indep_u32_kernel(uint32_t* __restrict__ W
     indep_tile_u32<T>(W, ldw, kb, bi * kMacro, bj * kMacro, lds);
 }
 
+// Strip launch of the 16-bit look-ahead schedule in 64 x 64 quarter tiles: the 2 nb - 1
+// macro tiles of macro row/column mnext, four quarters each (workgroup 4 t + q: quarter
+// q = 2 qi + qj of strip tile t, numbered as select_tile(1, ...) numbers them).
+template <uint32_t T, bool F16>
+__global__ __launch_bounds__(256) void strip64_u16_kernel(uint16_t* __restrict__ W, size_t ldw, uint32_t kb,
+                                                          uint32_t mnext, uint32_t nb) {
+    __shared__ __align__(16) uint16_t lds[indep64_u16_lds_elems<T>()];
+    const uint32_t t = blockIdx.x >> 2, q = blockIdx.x & 3;
+    uint32_t bi, bj;
+    if (t < nb) {
+        bi = mnext;
+        bj = t;
+    } else {
+        const uint32_t r = t - nb;
+        bi = r < mnext ? r : r + 1;
+        bj = mnext;
+    }
+    indep_tile64_u16<T, F16>(W, ldw, kb, bi * kMacro + 64 * (q >> 1), bj * kMacro + 64 * (q & 1), lds);
+}
+
 // ---------------------------------------------------------------------------------------
 // 16-bit look-ahead launch: workgroups [0, npanel) are the panel of the pivot block at
 // kb_next (one row chunk + one column chunk each, panel_u16), the others the independent
@@ -473,7 +493,7 @@ int run_rounds16(uint16_t* W, size_t ldw, uint32_t n, uint32_t N, uint32_t* pivo
         } else {
             const uint32_t mnext = (kb + T) / kMacro;
             tm.mark(kKindIndep);
-            launch_indep<uint16_t, T, F16>(W, ldw, kb, 1, mnext, nb, s);
+            strip64_u16_kernel<T, F16><<<4 * (2 * nb - 1), kThreads, 0, s>>>(W, ldw, kb, mnext, nb);
             panel_indep_u16_kernel<T, F16><<<npanel + nb * nb, kThreads, 0, s>>>(
                 W, ldw, kb, mnext, nb, npanel, kb + T, pivot_of(r + 1), pivot_of(r), kb);
             tm.mark(kKindIndep);

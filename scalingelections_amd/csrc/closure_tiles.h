@@ -328,6 +328,109 @@ __device__ __forceinline__ void indep_tile_u16(uint16_t* __restrict__ W, size_t 
         indep_tile_u16_impl<T, F16, true>(W, ldw, kb, I0, J0, lds);
 }
 
+// Quarter tiles (64 x 64) for the strip launch, which has only 2 nb - 1 macro tiles: four
+// times as many workgroups keep four times as many CUs busy on that short launch.  Lane
+// (ty, tx) owns rows ty + 16 r (r < 4) and columns 4 tx .. 4 tx + 3 (two packed pairs).
+// LDS: A strip 64 x (T + 8) and B strip T x 64 u16 (indep64_u16_lds_elems<T>()).
+template <uint32_t T>
+constexpr uint32_t indep64_u16_lds_elems() { return 64 * (T + 8) + T * 64; }
+
+template <uint32_t T, bool F16, bool MASKED>
+__device__ __forceinline__ void indep_tile64_u16_impl(uint16_t* __restrict__ W, size_t ldw, uint32_t kb,
+                                                      uint32_t I0, uint32_t J0, uint16_t* __restrict__ lds) {
+    constexpr uint32_t AP = T + 8;
+    uint16_t* As = lds;
+    uint16_t* Bs = lds + 64 * AP;
+    const uint32_t t = threadIdx.x, tx = t & 15, ty = t >> 4;
+    __syncthreads();
+
+    uint2 c[4];  // pivot rows/columns neither read nor stored (as in indep_tile_u16)
+    const uint16_t* cp = W + (size_t)(I0 + ty) * ldw + J0 + tx * 4;
+    const bool col_in_pivot = MASKED && J0 + tx * 4 - kb < T;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        if constexpr (MASKED) {
+            c[r] = make_uint2(0, 0);
+            if (!col_in_pivot && I0 + ty + 16 * r - kb >= T)
+                c[r] = *reinterpret_cast<const uint2*>(cp + (size_t)(16 * r) * ldw);
+        } else {
+            c[r] = *reinterpret_cast<const uint2*>(cp + (size_t)(16 * r) * ldw);
+        }
+    }
+    for (uint32_t q = t; q < 64 * (T / 8); q += kThreads) {
+        const uint32_t row = q / (T / 8), ch = q % (T / 8);
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (!MASKED || I0 + row - kb >= T)
+            v = *reinterpret_cast<const uint4*>(W + (size_t)(I0 + row) * ldw + kb + ch * 8);
+        *reinterpret_cast<uint4*>(As + row * AP + ch * 8) = v;
+    }
+    for (uint32_t q = t; q < T * 8; q += kThreads) {
+        const uint32_t row = q >> 3, ch = q & 7;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (!MASKED || J0 + ch * 8 - kb >= T)
+            v = *reinterpret_cast<const uint4*>(W + (size_t)(kb + row) * ldw + J0 + ch * 8);
+        *reinterpret_cast<uint4*>(Bs + row * 64 + ch * 8) = v;
+    }
+    __syncthreads();
+
+    u16x2 acc[4][2];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        acc[r][0] = as_u16x2(c[r].x);
+        acc[r][1] = as_u16x2(c[r].y);
+    }
+#pragma unroll 1
+    for (uint32_t k0 = 0; k0 < T; k0 += 8) {
+        uint4 a4[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) a4[r] = *reinterpret_cast<const uint4*>(As + (ty + 16 * r) * AP + k0);
+#pragma unroll
+        for (int kk = 0; kk < 8; kk += 2) {
+            const uint2 b4 = *reinterpret_cast<const uint2*>(Bs + (k0 + kk) * 64 + tx * 4);
+            const uint2 b5 = *reinterpret_cast<const uint2*>(Bs + (k0 + kk + 1) * 64 + tx * 4);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const uint32_t a2w = word(a4[r], kk >> 1);
+                if constexpr (F16) {
+                    const h16x2 a2 = as_h16x2(a2w);
+                    const h16x2 a0 = a2.xx, a1 = a2.yy;
+                    const h16x2 m00 = __builtin_elementwise_minimum(a0, as_h16x2(b4.x));
+                    const h16x2 m01 = __builtin_elementwise_minimum(a0, as_h16x2(b4.y));
+                    const h16x2 m10 = __builtin_elementwise_minimum(a1, as_h16x2(b5.x));
+                    const h16x2 m11 = __builtin_elementwise_minimum(a1, as_h16x2(b5.y));
+                    acc[r][0] = as_u16x2(h_as_u32(__builtin_elementwise_maximum(
+                        __builtin_elementwise_maximum(as_h16x2(as_u32(acc[r][0])), m00), m10)));
+                    acc[r][1] = as_u16x2(h_as_u32(__builtin_elementwise_maximum(
+                        __builtin_elementwise_maximum(as_h16x2(as_u32(acc[r][1])), m01), m11)));
+                } else {
+                    const u16x2 a2 = as_u16x2(a2w);
+                    const u16x2 a0 = a2.xx, a1 = a2.yy;
+                    acc[r][0] = pk_max(pk_max(acc[r][0], pk_min(a0, as_u16x2(b4.x))), pk_min(a1, as_u16x2(b5.x)));
+                    acc[r][1] = pk_max(pk_max(acc[r][1], pk_min(a0, as_u16x2(b4.y))), pk_min(a1, as_u16x2(b5.y)));
+                }
+            }
+        }
+    }
+
+    if (MASKED && J0 + tx * 4 - kb < T) return;
+    uint16_t* op = W + (size_t)(I0 + ty) * ldw + J0 + tx * 4;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        if (MASKED && I0 + ty + 16 * r - kb < T) continue;
+        *reinterpret_cast<uint2*>(op + (size_t)(16 * r) * ldw) = make_uint2(as_u32(acc[r][0]), as_u32(acc[r][1]));
+    }
+}
+
+template <uint32_t T, bool F16>
+__device__ __forceinline__ void indep_tile64_u16(uint16_t* __restrict__ W, size_t ldw, uint32_t kb, uint32_t I0,
+                                                 uint32_t J0, uint16_t* __restrict__ lds) {
+    const bool clean = (I0 + 64 <= kb || I0 >= kb + T) && (J0 + 64 <= kb || J0 >= kb + T);
+    if (clean)
+        indep_tile64_u16_impl<T, F16, false>(W, ldw, kb, I0, J0, lds);
+    else
+        indep_tile64_u16_impl<T, F16, true>(W, ldw, kb, I0, J0, lds);
+}
+
 // Independent tiles, 32-bit.  Lane (ty, tx) owns rows ty + 16 r (r < 8) and columns
 // 4 tx .. 4 tx + 3 and 64 + 4 tx .. 64 + 4 tx + 3 (each half is 256 contiguous bytes per
 // row across 16 lanes, so B-strip reads and global accesses stay conflict-free and
