@@ -293,7 +293,7 @@ __global__ __launch_bounds__(256) void panel_indep_u16_kernel(uint16_t* __restri
         // the panel is the round's critical path: its waves issue ahead of the tiles' waves
         // sharing their SIMDs (the tiles fill the issue slots the panel's barriers leave)
         __builtin_amdgcn_s_setprio(3);
-        panel_u16<T>(W, ldw, kb_next, blockIdx.x, pivot_out, pivot_prev, kb_prev, lds);
+        panel_u16<T, F16>(W, ldw, kb_next, blockIdx.x, pivot_out, pivot_prev, kb_prev, lds);
         return;
     }
     const uint32_t t = blockIdx.x - npanel;

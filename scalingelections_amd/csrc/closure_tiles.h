@@ -579,10 +579,24 @@ __device__ __forceinline__ void close_pivot_u16(const uint16_t* __restrict__ W, 
     __syncthreads();
 }
 
+// f16-ordered packed max(acc, min(a0, b0), min(a1, b1)) in 3 instructions (F16), or
+// the integer form in 4.
+template <bool F16>
+__device__ __forceinline__ u16x2 maxmin2(u16x2 acc, u16x2 a0, u16x2 b0, u16x2 a1, u16x2 b1) {
+    if constexpr (F16) {
+        const h16x2 m0 = __builtin_elementwise_minimum(as_h16x2(as_u32(a0)), as_h16x2(as_u32(b0)));
+        const h16x2 m1 = __builtin_elementwise_minimum(as_h16x2(as_u32(a1)), as_h16x2(as_u32(b1)));
+        return as_u16x2(h_as_u32(
+            __builtin_elementwise_maximum(__builtin_elementwise_maximum(as_h16x2(as_u32(acc)), m0), m1)));
+    } else {
+        return pk_max(pk_max(acc, pk_min(a0, b0)), pk_min(a1, b1));
+    }
+}
+
 // Row chunk (K, [c0, c0 + 128)): new[i][j] = max(old[i][j], max_k min(P[i][k], old[k][j])).
 // Lane (ty, tx): pivot rows R ty + r (r < R), columns c0 + 8 tx .. 8 tx + 7 (4 packed pairs).
 // S holds the chunk's old values (staged by the caller, followed by a barrier).
-template <uint32_t T>
+template <uint32_t T, bool F16>
 __device__ __forceinline__ void panel_rows_u16(uint16_t* __restrict__ W, size_t ldw, uint32_t kb, uint32_t c0,
                                                const uint16_t* P16, const uint16_t* S) {
     constexpr uint32_t R = T / 16, PP = T + 2;
@@ -604,8 +618,7 @@ __device__ __forceinline__ void panel_rows_u16(uint16_t* __restrict__ W, size_t 
             const u16x2 a0 = a2.xx, a1 = a2.yy;
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                acc[r][q] = pk_max(pk_max(acc[r][q], pk_min(a0, as_u16x2(word(b4, q)))),
-                                   pk_min(a1, as_u16x2(word(b5, q))));
+                acc[r][q] = maxmin2<F16>(acc[r][q], a0, as_u16x2(word(b4, q)), a1, as_u16x2(word(b5, q)));
         }
     }
     if (c0 + 8 * tx - kb < T) return;  // columns of the pivot block stay untouched
@@ -617,26 +630,14 @@ __device__ __forceinline__ void panel_rows_u16(uint16_t* __restrict__ W, size_t 
 
 // Column chunk ([c0, c0 + 128), K): new[i][j] = max(old[i][j], max_k min(old[i][k], P[k][j])).
 // Lane (ty, tx): rows c0 + 8 ty + r (r < 8), pivot columns R tx .. R tx + R - 1 (R / 2 pairs).
-template <uint32_t T>
+// S holds the chunk transposed (S[k][i] = old[c0 + i][kb + k], pitch 136), staged by the
+// caller and followed by a barrier.
+template <uint32_t T, bool F16>
 __device__ __forceinline__ void panel_cols_u16(uint16_t* __restrict__ W, size_t ldw, uint32_t kb, uint32_t c0,
-                                               const uint16_t* P16, uint16_t* S) {
+                                               const uint16_t* P16, const uint16_t* S) {
     constexpr uint32_t R = T / 16, PP = T + 2, SP = 136;
     static_assert(R % 2 == 0, "column chunk pairs need an even share of pivot columns");
     const uint32_t t = threadIdx.x, ty = t >> 4, tx = t & 15;
-    __syncthreads();  // S held the row chunk
-    // transposed staging: thread (k, 8-row group) gathers 8 rows of column kb + k
-    for (uint32_t q = t; q < T * 16; q += kThreads) {
-        const uint32_t k = q % T, g = q / T;
-        uint32_t w[4];
-#pragma unroll
-        for (int h = 0; h < 4; ++h) {
-            const uint32_t lo = W[(size_t)(c0 + 8 * g + 2 * h) * ldw + kb + k];
-            const uint32_t hi = W[(size_t)(c0 + 8 * g + 2 * h + 1) * ldw + kb + k];
-            w[h] = lo | (hi << 16);
-        }
-        *reinterpret_cast<uint4*>(S + k * SP + 8 * g) = make_uint4(w[0], w[1], w[2], w[3]);
-    }
-    __syncthreads();
     u16x2 acc[8][R / 2];
 #pragma unroll
     for (uint32_t c = 0; c < R; c += 2) {
@@ -650,18 +651,21 @@ __device__ __forceinline__ void panel_cols_u16(uint16_t* __restrict__ W, size_t 
         }
     }
 #pragma unroll 2
-    for (uint32_t k = 0; k < T; ++k) {
+    for (uint32_t k = 0; k < T; k += 2) {
         const uint4 a4 = *reinterpret_cast<const uint4*>(S + k * SP + 8 * ty);
-        u16x2 b[R / 2];
+        const uint4 a5 = *reinterpret_cast<const uint4*>(S + (k + 1) * SP + 8 * ty);
+        u16x2 b0[R / 2], b1[R / 2];
 #pragma unroll
-        for (uint32_t c = 0; c < R / 2; ++c)
-            b[c] = as_u16x2(*reinterpret_cast<const uint32_t*>(P16 + k * PP + R * tx + 2 * c));
+        for (uint32_t c = 0; c < R / 2; ++c) {
+            b0[c] = as_u16x2(*reinterpret_cast<const uint32_t*>(P16 + k * PP + R * tx + 2 * c));
+            b1[c] = as_u16x2(*reinterpret_cast<const uint32_t*>(P16 + (k + 1) * PP + R * tx + 2 * c));
+        }
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
-            const u16x2 a2 = as_u16x2(word(a4, r >> 1));
-            const u16x2 av = (r & 1) ? a2.yy : a2.xx;
+            const u16x2 x2 = as_u16x2(word(a4, r >> 1)), y2 = as_u16x2(word(a5, r >> 1));
+            const u16x2 av = (r & 1) ? x2.yy : x2.xx, aw = (r & 1) ? y2.yy : y2.xx;
 #pragma unroll
-            for (uint32_t c = 0; c < R / 2; ++c) acc[r][c] = pk_max(acc[r][c], pk_min(av, b[c]));
+            for (uint32_t c = 0; c < R / 2; ++c) acc[r][c] = maxmin2<F16>(acc[r][c], av, b0[c], aw, b1[c]);
         }
     }
     if (c0 + 8 * ty - kb < T) return;  // rows of the pivot block stay untouched
@@ -679,15 +683,17 @@ __device__ __forceinline__ void panel_cols_u16(uint16_t* __restrict__ W, size_t 
 
 // The whole panel workgroup: commits the previous round's closed pivot (workgroup 0, when
 // given), closes this pivot, publishes it (workgroup 0), then row chunk and column chunk
-// `chunk`.  Starts and ends with a barrier.
-template <uint32_t T>
+// `chunk`.  Both chunks are read from W before the pivot closes (neither product writes a
+// cell the other reads: they meet only in the pivot block, which neither stores), so their
+// loads overlap the pivot closure.  Starts with a barrier.
+template <uint32_t T, bool F16>
 __device__ __forceinline__ void panel_u16(uint16_t* __restrict__ W, size_t ldw, uint32_t kb, uint32_t chunk,
                                           uint32_t* __restrict__ pivot_out, const uint32_t* __restrict__ pivot_prev,
                                           uint32_t kb_prev, uint16_t* lds) {
-    constexpr uint32_t NQ = T * 16 / kThreads;
+    constexpr uint32_t NQ = T * 16 / kThreads, SP = 136;
     uint16_t* P16 = lds;
     uint16_t* S = lds + T * (T + 2);
-    uint32_t* rowbuf = reinterpret_cast<uint32_t*>(S + T * 136);
+    uint32_t* rowbuf = reinterpret_cast<uint32_t*>(S + T * SP);
     uint32_t* colbuf = rowbuf + 2 * T;
     const uint32_t t = threadIdx.x, c0 = chunk * kMacro;
     __syncthreads();
@@ -695,18 +701,64 @@ __device__ __forceinline__ void panel_u16(uint16_t* __restrict__ W, size_t ldw, 
         for (uint32_t q = t; q < T * T; q += kThreads)
             W[(size_t)(kb_prev + q / T) * ldw + kb_prev + q % T] = (uint16_t)pivot_prev[q];
     }
-    // row chunk into S (not used by the pivot closure, whose barriers publish it)
+    // both chunks into registers; thread q of the column gather takes (k = q % T, rows
+    // c0 + 8 g .. 8 g + 7 with g = q / T) and packs them for the transposed layout
+    static_assert(NQ == 2 || NQ == 4, "chunk share per thread");
+    auto gather = [&](uint32_t i) {  // 8 rows of one pivot column, packed in pairs
+        const uint32_t q = t + i * kThreads, k = q % T, g = q / T;
+        uint32_t w[4];
 #pragma unroll
-    for (uint32_t i = 0; i < NQ; ++i) {
+        for (int h = 0; h < 4; ++h) {
+            const uint32_t lo = W[(size_t)(c0 + 8 * g + 2 * h) * ldw + kb + k];
+            const uint32_t hi = W[(size_t)(c0 + 8 * g + 2 * h + 1) * ldw + kb + k];
+            w[h] = lo | (hi << 16);
+        }
+        return make_uint4(w[0], w[1], w[2], w[3]);
+    };
+    auto put_col = [&](uint32_t i, const uint4& v) {
+        const uint32_t q = t + i * kThreads, k = q % T, g = q / T;
+        *reinterpret_cast<uint4*>(S + k * SP + 8 * g) = v;
+    };
+    auto load_row = [&](uint32_t i) {
         const uint32_t q = t + i * kThreads, row = q >> 4, ch = q & 15;
-        *reinterpret_cast<uint4*>(S + row * kMacro + ch * 8) =
-            *reinterpret_cast<const uint4*>(W + (size_t)(kb + row) * ldw + c0 + ch * 8);
+        return *reinterpret_cast<const uint4*>(W + (size_t)(kb + row) * ldw + c0 + ch * 8);
+    };
+    auto put_row = [&](uint32_t i, const uint4& v) {
+        const uint32_t q = t + i * kThreads, row = q >> 4, ch = q & 15;
+        *reinterpret_cast<uint4*>(S + row * kMacro + ch * 8) = v;
+    };
+    // named registers (not arrays, which the compiler keeps on the stack here): the values
+    // live across the fully unrolled pivot closure
+    const uint4 row0 = load_row(0), row1 = load_row(1);
+    uint4 row2 = make_uint4(0, 0, 0, 0), row3 = make_uint4(0, 0, 0, 0);
+    const uint4 col0 = gather(0), col1 = gather(1);
+    uint4 col2 = make_uint4(0, 0, 0, 0), col3 = make_uint4(0, 0, 0, 0);
+    if constexpr (NQ == 4) {
+        row2 = load_row(2);
+        row3 = load_row(3);
+        col2 = gather(2);
+        col3 = gather(3);
     }
-    close_pivot_u16<T>(W, ldw, kb, P16, rowbuf, colbuf);
+    close_pivot_u16<T>(W, ldw, kb, P16, rowbuf, colbuf);  // ends with a barrier
     if (chunk == 0)
         for (uint32_t q = t; q < T * T; q += kThreads) pivot_out[q] = P16[(q / T) * (T + 2) + q % T];
-    panel_rows_u16<T>(W, ldw, kb, c0, P16, S);
-    panel_cols_u16<T>(W, ldw, kb, c0, P16, S);
+    put_row(0, row0);
+    put_row(1, row1);
+    if constexpr (NQ == 4) {
+        put_row(2, row2);
+        put_row(3, row3);
+    }
+    __syncthreads();
+    panel_rows_u16<T, F16>(W, ldw, kb, c0, P16, S);
+    __syncthreads();  // S held the row chunk
+    put_col(0, col0);
+    put_col(1, col1);
+    if constexpr (NQ == 4) {
+        put_col(2, col2);
+        put_col(3, col3);
+    }
+    __syncthreads();
+    panel_cols_u16<T, F16>(W, ldw, kb, c0, P16, S);
 }
 
 }  // namespace se
