@@ -150,6 +150,10 @@ int se_rank_device(const uint32_t *paths_dev, uint32_t n, size_t row_stride, uin
  * workgroup per election for p0, closure, wins and ranking).  SE_BATCH_PER_LAUNCH runs
  * every election through the single-election path instead (for comparison). */
 #define SE_BATCH_PER_LAUNCH 64u
+/* Close every batched election in one workgroup, also those above 128 candidates, which by
+ * default share per-round launches (panels, then tiles) with the batch's other large
+ * elections.  Results never change; kept for comparison. */
+#define SE_BATCH_ONE_GROUP 1024u
 
 typedef struct se_batch_timing {
     double total_ms;          /* counting + closure                                    */

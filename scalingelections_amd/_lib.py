@@ -45,6 +45,7 @@ EXPORTED = (
 )
 
 SE_BATCH_PER_LAUNCH = 64
+SE_BATCH_ONE_GROUP = 1024
 
 
 class se_batch_timing(ctypes.Structure):

@@ -95,9 +95,9 @@ class DeviceBatch:
         self.C = np.ascontiguousarray(w.C, dtype=np.uint32)
         self.m = np.ascontiguousarray(w.m, dtype=np.uint32)
 
-    def run(self, per_launch: bool = False, stream=None) -> dict:
+    def run(self, per_launch: bool = False, stream=None, one_group: bool = False) -> dict:
         t = _lib.se_batch_timing()
-        flags = _lib.SE_BATCH_PER_LAUNCH if per_launch else 0
+        flags = (_lib.SE_BATCH_PER_LAUNCH if per_launch else 0) | (_lib.SE_BATCH_ONE_GROUP if one_group else 0)
         rc = _lib.lib().se_batch_device(self.C.shape[0], _lib.ptr(self.C), _lib.ptr(self.m), self.pos.ptr,
                                         self.paths.ptr, self.ranking.ptr, flags,
                                         ctypes.c_void_p(stream.handle) if stream else None, ctypes.byref(t))

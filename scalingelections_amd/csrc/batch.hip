@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 #include "closure_tiles.h"
@@ -50,7 +51,14 @@ struct ElectionMeta {
     uint64_t w_off;    // element offset of the work matrix (Cp x Cp)
     uint64_t r_off;    // element offset of the ranking (C uint32)
     uint32_t C, m, Cp, K;  // K: counter packing width (voters per word)
+    uint32_t piv;          // index among the round-closed elections (kNoRounds: closed alone)
+    uint32_t pad[3];
 };
+
+constexpr uint32_t kNoRounds = 0xFFFFFFFFu;
+// Elections above this many candidates are closed by launches shared with the other large
+// elections of the batch, one pair per round (panels, then tiles), on the 16-bit paths.
+constexpr uint32_t kRoundsMinC = 129;
 
 // ------------------------------------------------------------------------- counting
 // item.x = election, item.y = first row I0, item.z = first column J0 (candidates),
@@ -111,12 +119,15 @@ __device__ __forceinline__ void indep_tile(E* W, size_t ldw, uint32_t kb, uint32
 constexpr uint32_t kBatchF16Bias = 0x3FFu;
 constexpr uint32_t kBatchF16MaxVoters = 0x7C00u - 0x0400u;
 
+// phase 0: every election writes p0 into its work matrix; elections closed alone (piv ==
+// kNoRounds) also run the closure and the tail.  phase 1: the tail (output, wins, ranking)
+// of the round-closed elections.
 template <typename E, uint32_t T, bool F16>
 __global__ __launch_bounds__(256) void batch_closure_kernel(const uint32_t* __restrict__ d_all,
                                                             E* __restrict__ W_all, uint32_t* __restrict__ p_all,
                                                             uint32_t* __restrict__ rank_all,
                                                             const ElectionMeta* __restrict__ meta,
-                                                            const uint32_t* __restrict__ order) {
+                                                            const uint32_t* __restrict__ order, uint32_t phase) {
     __shared__ __align__(16) BatchLds<E, T> L;
     const uint32_t t = threadIdx.x;
     const ElectionMeta em = meta[order[blockIdx.x]];
@@ -125,9 +136,12 @@ __global__ __launch_bounds__(256) void batch_closure_kernel(const uint32_t* __re
     uint32_t* p = p_all + em.d_off;
     E* W = W_all + em.w_off;
     if (C == 0) return;
+    const bool alone = em.piv == kNoRounds;
+    constexpr uint32_t bias = F16 ? kBatchF16Bias : 0u;
+    if (phase == 1 && alone) return;
+    if (phase == 1) goto tail;
 
     // p0 (scaling_elections.py:103-112) into the padded work matrix
-    constexpr uint32_t bias = F16 ? kBatchF16Bias : 0u;
     for (uint32_t q = t; q < Cp * Cp; q += kThreads) {
         const uint32_t i = q / Cp, j = q % Cp;
         uint32_t v = 0;
@@ -138,9 +152,9 @@ __global__ __launch_bounds__(256) void batch_closure_kernel(const uint32_t* __re
         W[q] = (E)v;
     }
     __syncthreads();
+    if (!alone) return;
 
-    const uint32_t rounds = (C + T - 1) / T;
-    for (uint32_t r = 0; r < rounds; ++r) {
+    for (uint32_t r = 0, rounds = (C + T - 1) / T; r < rounds; ++r) {
         const uint32_t kb = r * T;
         close_pivot<E, T>(W, Cp, kb, min(T, C - kb), L.P, L.rowbuf, L.colbuf);
         if (Cp > T) {
@@ -164,6 +178,7 @@ __global__ __launch_bounds__(256) void batch_closure_kernel(const uint32_t* __re
         }
     }
 
+tail:
     // output with zero diagonal, then wins (:287-290) and the stable ranking (:292)
     for (uint32_t q = t; q < C * C; q += kThreads) {
         const uint32_t i = q / C, j = q % C;
@@ -186,6 +201,49 @@ __global__ __launch_bounds__(256) void batch_closure_kernel(const uint32_t* __re
         }
         rank_all[em.r_off + rank] = i;
     }
+}
+
+// Round r of the round-closed elections, 16-bit: one workgroup per election for the
+// round's panel (panel_all_u16: the pivot closed once in registers, then every row chunk
+// and column chunk), then one workgroup per (election, macro tile) of the round's
+// independent tiles.  Each election keeps its own double-buffered pivot hand-off
+// (pivots + piv * 2 T^2); the panel of round r commits the pivot of round r - 1.
+template <uint32_t T, bool F16>
+__global__ __launch_bounds__(256) void batch_panel_kernel(uint16_t* __restrict__ W_all,
+                                                          const ElectionMeta* __restrict__ meta,
+                                                          const uint2* __restrict__ items, uint32_t r,
+                                                          uint32_t* __restrict__ pivots) {
+    __shared__ __align__(16) uint16_t lds[panel_u16_lds_elems<T>()];
+    const uint2 it = items[blockIdx.x];  // (election, unused)
+    const ElectionMeta em = meta[it.x];
+    uint32_t* pv = pivots + (size_t)em.piv * 2 * T * T;
+    const uint32_t kb = r * T;
+    panel_all_u16<T, F16>(W_all + em.w_off, em.Cp, kb, em.Cp / kMacro, pv + (r & 1) * T * T,
+                          r ? pv + ((r - 1) & 1) * T * T : nullptr, r ? kb - T : 0u, lds);
+}
+
+template <uint32_t T, bool F16>
+__global__ __launch_bounds__(256) void batch_tile_kernel(uint16_t* __restrict__ W_all,
+                                                         const ElectionMeta* __restrict__ meta,
+                                                         const uint4* __restrict__ items, uint32_t r) {
+    __shared__ __align__(16) uint16_t lds[indep_u16_lds_elems<T>()];
+    const uint4 it = items[blockIdx.x];  // (election, I0, J0)
+    const ElectionMeta em = meta[it.x];
+    indep_tile_u16<T, F16>(W_all + em.w_off, em.Cp, r * T, it.y, it.z, lds);
+}
+
+// After the last round: every round-closed election commits its last closed pivot.
+template <uint32_t T>
+__global__ __launch_bounds__(256) void batch_commit_kernel(uint16_t* __restrict__ W_all,
+                                                           const ElectionMeta* __restrict__ meta,
+                                                           const uint32_t* __restrict__ list,
+                                                           const uint32_t* __restrict__ pivots) {
+    const ElectionMeta em = meta[list[blockIdx.x]];
+    const uint32_t last = (em.C + T - 1) / T - 1, kb = last * T;
+    const uint32_t* pv = pivots + (size_t)em.piv * 2 * T * T + (last & 1) * T * T;
+    uint16_t* W = W_all + em.w_off;
+    for (uint32_t q = threadIdx.x; q < T * T; q += kThreads)
+        W[(size_t)(kb + q / T) * em.Cp + kb + q % T] = (uint16_t)pv[q];
 }
 
 inline uint32_t padded_side(uint32_t C, uint32_t T) { return C <= T ? T : (uint32_t)round_up(C, kMacro); }
@@ -258,6 +316,8 @@ int batch_device_impl(DeviceContext* ctx, uint32_t E, const uint32_t* Cs, const 
             em.m = ms[e];
             em.Cp = em.C ? padded_side(em.C, T) : 0;
             em.K = count_pack(em.C, em.m);
+            em.piv = kNoRounds;
+            em.pad[0] = em.pad[1] = em.pad[2] = 0;
             em.pos_off = po;
             em.d_off = doff;
             em.w_off = wo;
@@ -306,6 +366,16 @@ int batch_device_impl(DeviceContext* ctx, uint32_t E, const uint32_t* Cs, const 
         std::iota(order.begin(), order.end(), 0u);
         std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return Cs[a] > Cs[b]; });
 
+        // elections closed by shared per-round launches (16-bit paths, above 128 candidates)
+        const bool rounds_mode = use16 && !(flags & SE_BATCH_ONE_GROUP);
+        std::vector<uint32_t> large;
+        if (rounds_mode)
+            for (uint32_t e = 0; e < E; ++e)
+                if (Cs[e] >= kRoundsMinC) {
+                    meta[e].piv = (uint32_t)large.size();
+                    large.push_back(e);
+                }
+        std::sort(large.begin(), large.end(), [&](uint32_t a, uint32_t b) { return Cs[a] > Cs[b]; });
         const size_t meta_bytes = round_up(E * sizeof(ElectionMeta), 256);
         const size_t item_bytes = round_up(std::max<size_t>(1, sorted_items.size()) * sizeof(uint4), 256);
         const size_t order_bytes = round_up((size_t)E * 4, 256);
@@ -328,17 +398,74 @@ int batch_device_impl(DeviceContext* ctx, uint32_t E, const uint32_t* Cs, const 
         if (!sorted_items.empty())
             batch_count_kernel<<<(uint32_t)sorted_items.size(), kCThreads, 0, s>>>(pos, dbuf, dmeta, ditems);
         SE_CHECK_LAUNCH("batch counter launch failed");
+        // work lists of the round-closed elections, built while the counter runs
+        uint32_t max_rounds = 0;
+        for (uint32_t e : large) max_rounds = std::max(max_rounds, (Cs[e] + T - 1) / T);
+        std::vector<uint2> panel_items;
+        std::vector<uint4> tile_items;
+        std::vector<size_t> panel_off(max_rounds + 1), tile_off(max_rounds + 1);
+        for (uint32_t r = 0; r < max_rounds; ++r) {
+            panel_off[r] = panel_items.size();
+            tile_off[r] = tile_items.size();
+            for (uint32_t e : large) {
+                if ((Cs[e] + T - 1) / T <= r) continue;
+                const uint32_t nbe = meta[e].Cp / kMacro;
+                panel_items.push_back(make_uint2(e, 0));
+                for (uint32_t I = 0; I < nbe; ++I)
+                    for (uint32_t J = 0; J < nbe; ++J) tile_items.push_back(make_uint4(e, I * kMacro, J * kMacro, 0));
+            }
+        }
+        panel_off[max_rounds] = panel_items.size();
+        tile_off[max_rounds] = tile_items.size();
+
+        uint32_t* pivots = nullptr;
+        uint2* dpanels = nullptr;
+        uint4* dtiles = nullptr;
+        uint32_t* dlarge = nullptr;
+        if (!large.empty()) {
+            const size_t tb = round_up(tile_items.size() * sizeof(uint4), 256);
+            const size_t pb = round_up(panel_items.size() * sizeof(uint2), 256);
+            const size_t lb = round_up(large.size() * 4, 256);
+            pivots = static_cast<uint32_t*>(scratch(ctx, kSlotBatchPivot, large.size() * 2 * T * T * 4));
+            char* rbuf = static_cast<char*>(scratch(ctx, kSlotBatchRounds, tb + pb + lb));
+            if (!pivots || !rbuf) return SE_ERR_ALLOC;
+            dtiles = reinterpret_cast<uint4*>(rbuf);
+            dpanels = reinterpret_cast<uint2*>(rbuf + tb);
+            dlarge = reinterpret_cast<uint32_t*>(rbuf + tb + pb);
+            if (!hip_ok(hipMemcpyAsync(dtiles, tile_items.data(), tile_items.size() * sizeof(uint4),
+                                       hipMemcpyHostToDevice, s),
+                        "Failed to copy data from host to device") ||
+                !hip_ok(hipMemcpyAsync(dpanels, panel_items.data(), panel_items.size() * sizeof(uint2),
+                                       hipMemcpyHostToDevice, s),
+                        "Failed to copy data from host to device") ||
+                !hip_ok(hipMemcpyAsync(dlarge, large.data(), large.size() * 4, hipMemcpyHostToDevice, s),
+                        "Failed to copy data from host to device"))
+                return SE_ERR_COPY;
+        }
         if (timing) (void)hipEventRecord(ev[1], s);
         if (E) {
+            uint16_t* W16 = static_cast<uint16_t*>(wbuf);
+            auto rounds16 = [&](auto f16_tag) {
+                constexpr bool F16 = decltype(f16_tag)::value;
+                batch_closure_kernel<uint16_t, 64, F16><<<E, kThreads, 0, s>>>(dbuf, W16, paths, ranking, dmeta, dorder, 0u);
+                if (large.empty()) return;
+                for (uint32_t r = 0; r < max_rounds; ++r) {
+                    const uint32_t np = (uint32_t)(panel_off[r + 1] - panel_off[r]);
+                    const uint32_t nt = (uint32_t)(tile_off[r + 1] - tile_off[r]);
+                    batch_panel_kernel<64, F16><<<np, kThreads, 0, s>>>(W16, dmeta, dpanels + panel_off[r], r, pivots);
+                    batch_tile_kernel<64, F16><<<nt, kThreads, 0, s>>>(W16, dmeta, dtiles + tile_off[r], r);
+                }
+                batch_commit_kernel<64><<<(uint32_t)large.size(), kThreads, 0, s>>>(W16, dmeta, dlarge, pivots);
+                batch_closure_kernel<uint16_t, 64, F16>
+                    <<<(uint32_t)large.size(), kThreads, 0, s>>>(dbuf, W16, paths, ranking, dmeta, dlarge, 1u);
+            };
             if (use16 && max_m <= kBatchF16MaxVoters)
-                batch_closure_kernel<uint16_t, 64, true>
-                    <<<E, kThreads, 0, s>>>(dbuf, static_cast<uint16_t*>(wbuf), paths, ranking, dmeta, dorder);
+                rounds16(std::true_type{});
             else if (use16)
-                batch_closure_kernel<uint16_t, 64, false>
-                    <<<E, kThreads, 0, s>>>(dbuf, static_cast<uint16_t*>(wbuf), paths, ranking, dmeta, dorder);
+                rounds16(std::false_type{});
             else
-                batch_closure_kernel<uint32_t, 32, false>
-                    <<<E, kThreads, 0, s>>>(dbuf, static_cast<uint32_t*>(wbuf), paths, ranking, dmeta, dorder);
+                batch_closure_kernel<uint32_t, 32, false><<<E, kThreads, 0, s>>>(
+                    dbuf, static_cast<uint32_t*>(wbuf), paths, ranking, dmeta, dorder, 0u);
         }
         SE_CHECK_LAUNCH("batch closure launch failed");
         if (timing) (void)hipEventRecord(ev[2], s);

@@ -761,4 +761,54 @@ __device__ __forceinline__ void panel_u16(uint16_t* __restrict__ W, size_t ldw, 
     panel_cols_u16<T, F16>(W, ldw, kb, c0, P16, S);
 }
 
+// The panel of one pivot block in one workgroup: the pivot is closed once, then every
+// row chunk and column chunk of the nb = ldw / 128 chunks in turn (for small matrices,
+// where one workgroup per chunk would close the same pivot nb times).  Same LDS layout
+// and hand-off as panel_u16.
+template <uint32_t T, bool F16>
+__device__ __forceinline__ void panel_all_u16(uint16_t* __restrict__ W, size_t ldw, uint32_t kb, uint32_t nb,
+                                              uint32_t* __restrict__ pivot_out,
+                                              const uint32_t* __restrict__ pivot_prev, uint32_t kb_prev,
+                                              uint16_t* lds) {
+    constexpr uint32_t NQ = T * 16 / kThreads, SP = 136;
+    uint16_t* P16 = lds;
+    uint16_t* S = lds + T * (T + 2);
+    uint32_t* rowbuf = reinterpret_cast<uint32_t*>(S + T * SP);
+    uint32_t* colbuf = rowbuf + 2 * T;
+    const uint32_t t = threadIdx.x;
+    __syncthreads();
+    if (pivot_prev != nullptr) {
+        for (uint32_t q = t; q < T * T; q += kThreads)
+            W[(size_t)(kb_prev + q / T) * ldw + kb_prev + q % T] = (uint16_t)pivot_prev[q];
+    }
+    close_pivot_u16<T>(W, ldw, kb, P16, rowbuf, colbuf);  // ends with a barrier
+    for (uint32_t q = t; q < T * T; q += kThreads) pivot_out[q] = P16[(q / T) * (T + 2) + q % T];
+    for (uint32_t chunk = 0; chunk < nb; ++chunk) {
+        const uint32_t c0 = chunk * kMacro;
+#pragma unroll
+        for (uint32_t i = 0; i < NQ; ++i) {
+            const uint32_t q = t + i * kThreads, row = q >> 4, ch = q & 15;
+            *reinterpret_cast<uint4*>(S + row * kMacro + ch * 8) =
+                *reinterpret_cast<const uint4*>(W + (size_t)(kb + row) * ldw + c0 + ch * 8);
+        }
+        __syncthreads();
+        panel_rows_u16<T, F16>(W, ldw, kb, c0, P16, S);
+        __syncthreads();
+        for (uint32_t q = t; q < T * 16; q += kThreads) {
+            const uint32_t k = q % T, g = q / T;
+            uint32_t w[4];
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                const uint32_t lo = W[(size_t)(c0 + 8 * g + 2 * h) * ldw + kb + k];
+                const uint32_t hi = W[(size_t)(c0 + 8 * g + 2 * h + 1) * ldw + kb + k];
+                w[h] = lo | (hi << 16);
+            }
+            *reinterpret_cast<uint4*>(S + k * SP + 8 * g) = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+        __syncthreads();
+        panel_cols_u16<T, F16>(W, ldw, kb, c0, P16, S);
+        __syncthreads();
+    }
+}
+
 }  // namespace se

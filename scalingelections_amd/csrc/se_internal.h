@@ -34,7 +34,9 @@ enum Slot : int {
     kSlotBatchMeta = 7,  // batch metadata / per-launch wins
     kSlotCountPart = 8,  // per-slice partial preference matrices
     kSlotCountFlag = 9,  // counter range-check flag
-    kSlotCount = 10
+    kSlotBatchPivot = 10,   // per-election pivot hand-off of the round-closed batch elections
+    kSlotBatchRounds = 11,  // their per-round work lists
+    kSlotCount = 12
 };
 
 struct DeviceContext {

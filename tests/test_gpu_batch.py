@@ -21,17 +21,24 @@ def _oracle_election(oracle_mod, pos, C):
 
 
 def _check(oracle_mod, w, label, sample=None):
+    """Batched (large elections closed by shared per-round launches), batched with every
+    election in one workgroup, and one launch sequence per election must agree bit for bit;
+    a sample (or all) is also checked against the oracle."""
     from scalingelections_amd import batch
 
     db = batch.DeviceBatch(w)
     db.run(per_launch=False)
     pb, rb = db.results()
+    db.run(per_launch=False, one_group=True)
+    pg, rg = db.results()
     db.run(per_launch=True)
     pl, rl = db.results()
     idx = range(w.elections) if sample is None else sample
     for e in range(w.elections):
         assert np.array_equal(pb[e], pl[e]), (label, "paths batched vs per-launch", e, int(w.C[e]))
         assert np.array_equal(rb[e], rl[e]), (label, "ranking batched vs per-launch", e, int(w.C[e]))
+        assert np.array_equal(pg[e], pl[e]), (label, "paths one-group vs per-launch", e, int(w.C[e]))
+        assert np.array_equal(rg[e], rl[e]), (label, "ranking one-group vs per-launch", e, int(w.C[e]))
     for e in idx:
         _, p, r = _oracle_election(oracle_mod, w.election_positions(e), int(w.C[e]))
         assert np.array_equal(pb[e], p), (label, "paths vs oracle", e, int(w.C[e]))
