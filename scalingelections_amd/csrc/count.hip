@@ -96,9 +96,9 @@ void launch_count(uint32_t grid, hipStream_t s, const int32_t* pos, uint32_t v0,
 }  // namespace
 
 // Counter plan: packing width K from n (bytes up to 128 candidates, 16-bit halves up to
-// 32768, int32 above), voter slices for parallelism (about 1024 workgroups, at least 32
-// voters each) capped at the largest slice a K-packed accumulation keeps exact, and passes
-// of at most `budget` bytes of partial planes, each pass adding into d.
+// 32768, int32 above), voter slices for parallelism (see below) capped at the largest
+// slice a K-packed accumulation keeps exact, and passes of at most `budget` bytes of
+// partial planes, each pass adding into d.
 int count_device_impl(DeviceContext* ctx, const int32_t* pos, uint32_t m, uint32_t n, uint32_t* d, size_t ldd,
                       hipStream_t s, double* kernel_ms, uint32_t* flag) {
     if (n == 0) return SE_OK;
@@ -110,8 +110,14 @@ int count_device_impl(DeviceContext* ctx, const int32_t* pos, uint32_t m, uint32
     uint64_t budget = 1ull << 30;
     if (const char* env = std::getenv("SE_COUNT_PARTIAL_BYTES")) budget = std::max<uint64_t>(1, std::strtoull(env, nullptr, 10));
     const uint64_t fit = std::max<uint64_t>(1, budget / plane_bytes);
-    uint64_t want = (1024 + tiles - 1) / tiles;
-    want = std::min<uint64_t>(want, std::max<uint32_t>(1, m / 32));
+    // voter slices: about 2560 workgroups (10 per CU, so the last wave is a small share)
+    // while every slice keeps at least 512 voters; otherwise about 1024 workgroups with
+    // at least 32 voters per slice
+    uint64_t want = (2560 + tiles - 1) / tiles;
+    if (m / want < 512) {
+        want = (1024 + tiles - 1) / tiles;
+        want = std::min<uint64_t>(want, std::max<uint32_t>(1, m / 32));
+    }
     want = std::min<uint64_t>(want, fit);
     want = std::max<uint64_t>(want, 1);
     uint64_t slice_len = m ? (m + want - 1) / want : 1;
