@@ -89,7 +89,51 @@ def cpu_baseline(d: np.ndarray, budget_s: float = 15.0) -> dict:
         "kind": "port",
         "sample": f"first {rounds} of {(n + T - 1) // T} pivot rounds (tile 16) of the same {n}-candidate "
                   f"matrix, {cells:.3e} cell updates in {el:.1f}s (p0 init excluded)",
+        "host": host_cpu(),
+        "small_configs": small_config_baselines(threads),
     }
+
+
+def host_cpu() -> dict:
+    """CPU model and logical CPU count of the host the baseline ran on."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "os_cpu_count": os.cpu_count()}
+
+
+def small_config_baselines(threads: int) -> dict:
+    """The 128 x 2000 election (serial and tiled oracle, whole closure) and the 4096 x 4096
+    election (tiled oracle, whole closure), each beside the GPU closure of the same d."""
+    import oracle
+    from scalingelections_amd import api, synth
+
+    out = {}
+    for name, m, n, seed, serial in (("128x2000", 2000, 128, 1, True), ("4096x4096", 4096, 4096, 2, False)):
+        R = synth.impartial_culture(m, n, seed)
+        d = oracle.build_prefs(list(R))
+        row = {}
+        if serial:
+            t0 = time.time()
+            p_serial = oracle.closure_serial(d)
+            row["cpu_serial_s"] = round(time.time() - t0, 4)
+        t0 = time.time()
+        p_tiled = oracle.closure_tiled(d, 16, threads=threads)
+        row["cpu_tiled_s"] = round(time.time() - t0, 4)
+        api.compute_strongest_paths(d, allow_gpu=True, tile_size=32)
+        t0 = time.time()
+        p_gpu = api.compute_strongest_paths(d, allow_gpu=True, tile_size=32)
+        row["gpu_end_to_end_s"] = round(time.time() - t0, 5)
+        row["cpu_tiled_threads"] = threads
+        row["bit_exact"] = bool(np.array_equal(p_gpu, p_tiled) and (not serial or np.array_equal(p_gpu, p_serial)))
+        out[name] = row
+        log(f"small config {name}: {row}")
+    return out
 
 
 def recorded_traffic(mode: str, n: int):
