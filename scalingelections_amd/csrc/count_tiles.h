@@ -115,41 +115,78 @@ __device__ __forceinline__ uint32_t finish(int32_t acc, uint32_t si, uint32_t sj
     }
 }
 
+// Staging assignment of a thread: side (0 rows, 1 columns), candidate c of that side and
+// word phase: the 128 threads of a side split r candidates G = 128 / r ways, thread
+// (c, sub) staging words sub, sub + G, ... (all 256 threads load when r is small).
+struct StageLane {
+    uint32_t side, c, sub, G;
+    bool live;
+};
+
+__device__ __forceinline__ StageLane stage_lane(uint32_t ri, uint32_t rj, bool split) {
+    StageLane L;
+    const uint32_t t = threadIdx.x, local = t & (kCT - 1);
+    L.side = t >> 7;
+    const uint32_t r = L.side ? rj : ri;
+    if (!split) {  // thread = candidate
+        L.G = 1u;
+        L.c = local;
+        L.sub = 0u;
+        L.live = local < r;
+        return L;
+    }
+    L.G = r ? max(1u, kCT / r) : 1u;
+    L.c = r ? local % r : 0u;
+    L.sub = r ? local / r : 0u;
+    L.live = r != 0 && L.sub < L.G;
+    return L;
+}
+
 // Stages words [w0, w0 + kVW) of the slice (voters vb..ve, K per word, zero padded) for
-// rows I0 + [0, ri) into PI (guarded) and columns J0 + [0, rj) into PJ.  Thread t < 128
-// stages row candidate t, thread 128 + t column candidate t; each adds its positions to
-// `s` (the per-candidate sum) and ORs them into `bits` (range check).
-template <uint32_t K>
+// rows I0 + [0, ri) into PI (guarded) and columns J0 + [0, rj) into PJ; columns of the
+// staging arrays past ri / rj are left as they are (their cells are never stored).  Each
+// thread adds its positions to `s` (its share of the candidate's sum) and ORs them into
+// `bits` (range check).
+// SPLIT = false: one thread per candidate (G = 1), the loop the full tiles use.
+template <uint32_t K, bool SPLIT>
 __device__ __forceinline__ void stage_words(const int32_t* __restrict__ pos, size_t ldp, uint32_t vb, uint32_t ve,
-                                            uint32_t w0, uint32_t I0, uint32_t ri, uint32_t J0, uint32_t rj,
+                                            uint32_t w0, uint32_t I0, uint32_t J0, const StageLane& L,
                                             uint32_t (*PI)[kCT], uint32_t (*PJ)[kCT], uint32_t& s,
                                             uint32_t& bits) {
-    const uint32_t t = threadIdx.x, side = t >> 7, c = t & (kCT - 1);
-    const uint32_t base = side ? J0 : I0, r = side ? rj : ri;
-    uint32_t(*P)[kCT] = side ? PJ : PI;
-    const uint32_t guard = side ? 0u : PackTraits<K>::guard;
-    const bool live = c < r;
-    const int32_t* col = pos + base + c;
+    if (!L.live) return;
+    const uint32_t base = L.side ? J0 : I0;
+    uint32_t(*P)[kCT] = L.side ? PJ : PI;
+    const uint32_t guard = L.side ? 0u : PackTraits<K>::guard;
+    const int32_t* col = pos + base + L.c;
+    const uint32_t first = SPLIT ? L.sub : 0u, step = SPLIT ? L.G : 1u;
 #pragma unroll 4
-    for (uint32_t w = 0; w < kVW; ++w) {
+    for (uint32_t w = first; w < kVW; w += step) {
         uint32_t word = 0;
-        if (live) {
-            const uint32_t v0 = vb + (w0 + w) * K;
+        const uint32_t v0 = vb + (w0 + w) * K;
 #pragma unroll
-            for (uint32_t k = 0; k < K; ++k) {
-                const uint32_t v = v0 + k;
-                const uint32_t p = v < ve ? (uint32_t)col[(size_t)v * ldp] : 0u;
-                if constexpr (K > 1) {
-                    word |= p << (PackTraits<K>::shift * k);
-                    s += p;
-                    bits |= p;
-                } else {
-                    word = p;
-                }
+        for (uint32_t k = 0; k < K; ++k) {
+            const uint32_t v = v0 + k;
+            const uint32_t p = v < ve ? (uint32_t)col[(size_t)v * ldp] : 0u;
+            if constexpr (K > 1) {
+                word |= p << (PackTraits<K>::shift * k);
+                s += p;
+                bits |= p;
+            } else {
+                word = p;
             }
         }
-        P[w][c] = word | guard;
+        P[w][L.c] = word | guard;
     }
+}
+
+// Per-candidate position sums from the threads' shares: sums[side * 128 + c] (256 words of
+// LDS, not aliasing the staging arrays' live data).  Ends with a barrier.
+__device__ __forceinline__ void reduce_sums(uint32_t* sums, const StageLane& L, uint32_t s) {
+    __syncthreads();
+    sums[threadIdx.x] = 0u;
+    __syncthreads();
+    if (L.live) atomicAdd(&sums[L.side * kCT + L.c], s);
+    __syncthreads();
 }
 
 // d tile (I, J), I <= J, full 128 x 128, of one voter slice [vb, ve) of an election with
@@ -175,10 +212,11 @@ __device__ __forceinline__ uint32_t count_tile(const int32_t* __restrict__ pos, 
 #pragma unroll
         for (int b = 0; b < 8; ++b) acc[a][b] = 0;
     uint32_t s = 0, bits = 0;
+    const StageLane SL = stage_lane(ri, rj, false);
 
     for (uint32_t w0 = 0; w0 < words; w0 += kVW) {
         __syncthreads();
-        stage_words<K>(pos, ldp, vb, ve, w0, I0, ri, J0, rj, PI, PJ, s, bits);
+        stage_words<K, false>(pos, ldp, vb, ve, w0, I0, J0, SL, PI, PJ, s, bits);
         __syncthreads();
         const uint32_t wend = min(kVW, words - w0);
         for (uint32_t w = 0; w < wend; ++w) {
@@ -192,9 +230,7 @@ __device__ __forceinline__ uint32_t count_tile(const int32_t* __restrict__ pos, 
             for (int a = 0; a < 8; ++a) tally_row<K>(acc[a], x[a], y);
         }
     }
-    __syncthreads();
-    red[threadIdx.x] = s;  // row sums in red[0, 128), column sums in red[128, 256)
-    __syncthreads();
+    reduce_sums(red, SL, s);  // row sums in red[0, 128), column sums in red[128, 256)
 
     const uint32_t M = words * K, mreal = ve - vb;
     const bool full = ri == kCT && rj == kCT && (ldd & 3) == 0 && ((uintptr_t)d & 15) == 0;
@@ -286,9 +322,10 @@ __device__ __forceinline__ uint32_t count_block(const int32_t* __restrict__ pos,
 #pragma unroll
         for (int c = 0; c < 8; ++c) acc[a][c] = 0;
     uint32_t s = 0, bits = 0;
+    const StageLane SL = stage_lane(ri, rj, true);
     for (uint32_t w0 = 0; w0 < words; w0 += kVW) {
         __syncthreads();
-        stage_words<K>(pos, ldp, vb, ve, w0, I0, ri, J0, rj, PI, PJ, s, bits);
+        stage_words<K, true>(pos, ldp, vb, ve, w0, I0, J0, SL, PI, PJ, s, bits);
         __syncthreads();
         if (g < G) {
             const uint32_t wend = min(kVW, words - w0);
@@ -305,8 +342,7 @@ __device__ __forceinline__ uint32_t count_block(const int32_t* __restrict__ pos,
         }
     }
     uint32_t* sums = red + kCThreads * 8;  // row sums [0, 128), column sums [128, 256)
-    __syncthreads();
-    sums[t] = s;
+    reduce_sums(sums, SL, s);
     const uint32_t M = words * K, mreal = ve - vb;
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
