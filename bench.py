@@ -13,6 +13,8 @@ Also reported, all measured in this run:
     launch on the library stream during the timed steps, against the packed 16-bit VALU peak;
   * end_to_end: the reference's convention (host upload + closure + download) with a
     sha256 check of p against the committed oracle fixture;
+  * closure_u32: the same 32768 matrix through the 32-bit closure (SE_FLAG_FORCE_U32, the
+    4 GB variant), against the 32-bit VALU bound, with the same sha256 check;
   * closure_by_size: device TCells/s at 4096, 8192 and 16384 candidates;
   * counter: 4096 ballots x 4096 candidates (seed 2), pair-votes/s against the VALU issue
     bound of the packed count (2 instructions per 2 pair-votes), and the whole 4096 x 4096
@@ -21,7 +23,7 @@ Also reported, all measured in this run:
     [100, 10000], seed 4), elections/s batched vs one launch sequence per election, with
     every election's paths and ranking compared between the two;
   * serving: Poisson arrivals from that mix plus 1% 8192 x 8192 Mallows elections, offered
-    load set to 60% of the capacity measured above, p50/p99 latency, every result checked
+    load set to 50% of the capacity measured above, p50/p99 latency, every result checked
     against the same election processed alone;
   * cpu_baseline: the CPU oracle (tiled restatement of the reference's Numba kernel,
     T = 16) on the 32768 matrix for a bounded number of pivot rounds.
@@ -398,6 +400,28 @@ def main() -> int:
     cells_per_round = indep_cells / max(1, rounds_run)
     achieved = cells_per_round / (avg_round_ms / 1e3) / 1e12 if avg_round_ms > 0 else 0.0
     peak = VALU_LANE_INSTR * UPDATES_PER_INSTR.get(mode, 1.0) / 1e12
+
+    u32 = None
+    if not args.no_extra:
+        # the same matrix through the 32-bit closure (the 4 GB variant of this workload)
+        log("32-bit closure of the same matrix (SE_FLAG_FORCE_U32): warmup")
+        _lib.check(L.se_closure_device(d_dev.ptr, n, n, p_dev.ptr, n, _lib.SE_FLAG_FORCE_U32, None, None))
+        devmem.device_synchronize()
+        t0 = time.perf_counter()
+        tm32 = closure_step_timed(L, _lib, d_dev.ptr, p_dev.ptr, n, _lib.SE_FLAG_FORCE_U32)
+        devmem.device_synchronize()
+        el32 = time.perf_counter() - t0
+        fixture = os.path.join(ROOT, "tests", "golden", f"c3_{n}_seed{args.seed}.json")
+        parity = "no fixture"
+        if os.path.exists(fixture):
+            got = p_dev.download(np.empty_like(d))
+            parity = ("bit-exact (sha256 of p matches the oracle fixture)"
+                      if sha(got) == json.load(open(fixture))["p_sha256"] else "MISMATCH")
+            del got
+        u32 = {"candidates": n, "ms": round(el32 * 1e3, 2), "tcells_per_s": round(float(n) ** 3 / el32 / 1e12, 3),
+               "roofline_frac": round(float(n) ** 3 / el32 / (VALU_LANE_INSTR * UPDATES_PER_INSTR["u32"]), 4),
+               "pivot_depth": tm32["pivot"], "width_bits": tm32["width_bits"], "parity": parity}
+        log(f"u32 closure {u32}")
     del p_dev
 
     e2e = None
@@ -418,6 +442,8 @@ def main() -> int:
         del out
 
     extra = {}
+    if u32 is not None:
+        extra["closure_u32"] = u32
     if not args.no_extra:
         extra["closure_by_size"] = closure_by_size([4096, 8192, 16384])
         extra["counter"] = counter_section()

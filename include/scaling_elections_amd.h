@@ -50,14 +50,13 @@ extern "C" {
 #define SE_FLAG_TIME_KERNELS 4u     /* record hipEvents around every kernel (se_timing)   */
 #define SE_FLAG_ALLOW_TMA 8u        /* accepted and ignored: AMD GPUs have no TMA         */
                                     /* (scaling_elections.cu:555-558)                     */
-#define SE_FLAG_PIVOT32 16u         /* force pivot depth 32 (default: 64 for 16-bit, 32    */
-#define SE_FLAG_PIVOT64 32u         /* for 32-bit); the result never depends on it         */
+#define SE_FLAG_PIVOT32 16u         /* force pivot depth 32 or 64 (default: the tuning     */
+#define SE_FLAG_PIVOT64 32u         /* table); the result never depends on it              */
 #define SE_FLAG_NO_F16 128u         /* 16-bit path: keep integer min/max even when the     */
                                     /* values fit the f16-ordered range                   */
-#define SE_FLAG_NO_LOOKAHEAD 256u   /* one stream, no overlap of the next round's pivot    */
-                                    /* and panels with the current independent tiles      */
-#define SE_FLAG_LOOKAHEAD 512u      /* overlap also below 8192 padded candidates (default  */
-                                    /* only from 8192 on)                                 */
+#define SE_FLAG_NO_LOOKAHEAD 256u   /* force off / on the overlap of the next round's      */
+#define SE_FLAG_LOOKAHEAD 512u      /* pivot and panels with the current independent tiles */
+                                    /* (default: the tuning table)                        */
 
 /* Per-call measurements, filled when a non-NULL pointer is passed. */
 typedef struct se_timing {
@@ -168,6 +167,35 @@ typedef struct se_batch_timing {
 int se_batch_device(uint32_t elections, const uint32_t *candidates, const uint32_t *voters,
                     const int32_t *positions_dev, uint32_t *paths_dev, uint32_t *ranking_dev,
                     uint32_t flags, void *stream, se_batch_timing *timing);
+
+/* ---- launch-shape tuning (no reference counterpart) ------------------------------------ */
+/* The closure's pivot depth and look-ahead, and the batch's split between elections closed
+ * by one workgroup and elections closed by shared per-round launches, are picked from a
+ * process-wide table.  The built-in table holds the choices measured fastest on MI355X;
+ * scalingelections_amd.autotune re-measures them on the current device and installs its own
+ * table.  Results never depend on the table: every choice computes the same closure.
+ * SE_FLAG_PIVOT32/64 and SE_FLAG_(NO_)LOOKAHEAD on a call override it. */
+#define SE_TUNE_F16 0u   /* 16-bit closure, f16-ordered (nonzero p0 values span <= 30720)   */
+#define SE_TUNE_U16 1u   /* 16-bit closure, integer min/max                                */
+#define SE_TUNE_U32 2u   /* 32-bit closure                                                 */
+#define SE_TUNE_MAX_ENTRIES 64u
+typedef struct se_tune_entry {
+    uint32_t width;      /* SE_TUNE_F16, SE_TUNE_U16 or SE_TUNE_U32                        */
+    uint32_t max_side;   /* applies to zero-padded sides N <= max_side (N: n rounded up to */
+                         /* 128); the smallest max_side >= N of the width is used          */
+    uint32_t pivot;      /* pivot depth: 32 or 64                                          */
+    uint32_t lookahead;  /* 1: overlap the next pivot's panel with the current round       */
+} se_tune_entry;
+/* Installs a table (entries need not be sorted); count 0 restores the built-in table.
+ * Widths without an entry covering N fall back to the built-in table. */
+int se_tune_closure(const se_tune_entry *entries, uint32_t count);
+/* Copies the active table (at most capacity entries); *count receives its size. */
+int se_tune_closure_get(se_tune_entry *entries, uint32_t capacity, uint32_t *count);
+/* Batched elections with at least rounds_min_candidates candidates share per-round launches
+ * (16-bit paths); smaller ones are closed by one workgroup each.  Accepted: 129..1025
+ * (1025: none share launches); 0 restores the built-in value. */
+int se_tune_batch(uint32_t rounds_min_candidates);
+int se_tune_batch_get(uint32_t *rounds_min_candidates);
 
 /* ---- workload generation (no reference counterpart) ------------------------------------ */
 /* Mallows ballots by repeated insertion (item k of the reference order inserted at

@@ -42,7 +42,13 @@ EXPORTED = (
     "se_rank", "se_rank_device", "se_reduce_f32",
     "se_interrupt", "se_clear_interrupt", "se_install_sigint", "se_restore_sigint",
     "se_batch_device", "se_synth_mallows", "se_stream_release",
+    "se_tune_closure", "se_tune_closure_get", "se_tune_batch", "se_tune_batch_get",
 )
+
+SE_TUNE_F16 = 0
+SE_TUNE_U16 = 1
+SE_TUNE_U32 = 2
+SE_TUNE_MAX_ENTRIES = 64
 
 SE_BATCH_PER_LAUNCH = 64
 SE_BATCH_ONE_GROUP = 1024
@@ -61,6 +67,15 @@ class se_batch_timing(ctypes.Structure):
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+class se_tune_entry(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_uint32),
+        ("max_side", ctypes.c_uint32),
+        ("pivot", ctypes.c_uint32),
+        ("lookahead", ctypes.c_uint32),
+    ]
 
 
 class se_timing(ctypes.Structure):
@@ -142,6 +157,14 @@ def _declare(L):
     L.se_synth_mallows.restype = c.c_int
     L.se_stream_release.argtypes = [vp]
     L.se_stream_release.restype = c.c_int
+    L.se_tune_closure.argtypes = [c.POINTER(se_tune_entry), u32]
+    L.se_tune_closure.restype = c.c_int
+    L.se_tune_closure_get.argtypes = [c.POINTER(se_tune_entry), u32, c.POINTER(u32)]
+    L.se_tune_closure_get.restype = c.c_int
+    L.se_tune_batch.argtypes = [u32]
+    L.se_tune_batch.restype = c.c_int
+    L.se_tune_batch_get.argtypes = [c.POINTER(u32)]
+    L.se_tune_batch_get.restype = c.c_int
     del i32, i64
 
 

@@ -56,9 +56,6 @@ struct ElectionMeta {
 };
 
 constexpr uint32_t kNoRounds = 0xFFFFFFFFu;
-// Elections above this many candidates are closed by launches shared with the other large
-// elections of the batch, one pair per round (panels, then tiles), on the 16-bit paths.
-constexpr uint32_t kRoundsMinC = 129;
 
 // ------------------------------------------------------------------------- counting
 // item.x = election, item.y = first row I0, item.z = first column J0 (candidates),
@@ -366,12 +363,15 @@ int batch_device_impl(DeviceContext* ctx, uint32_t E, const uint32_t* Cs, const 
         std::iota(order.begin(), order.end(), 0u);
         std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return Cs[a] > Cs[b]; });
 
-        // elections closed by shared per-round launches (16-bit paths, above 128 candidates)
+        // Elections with at least rounds_min candidates (tuning table, >= 129) are closed by
+        // launches shared with the other large elections of the batch, one pair per round
+        // (panels, then tiles), on the 16-bit paths; the others by one workgroup each.
         const bool rounds_mode = use16 && !(flags & SE_BATCH_ONE_GROUP);
+        const uint32_t rounds_min = tuned_batch_rounds_min_c();
         std::vector<uint32_t> large;
         if (rounds_mode)
             for (uint32_t e = 0; e < E; ++e)
-                if (Cs[e] >= kRoundsMinC) {
+                if (Cs[e] >= rounds_min) {
                     meta[e].piv = (uint32_t)large.size();
                     large.push_back(e);
                 }

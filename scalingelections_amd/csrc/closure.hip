@@ -592,9 +592,13 @@ int closure_device_ex(DeviceContext* ctx, const void* d_any, bool d16, uint32_t 
     tm.mark(kKindInit);
     SE_CHECK_LAUNCH("p0 init launch failed");
 
-    // pivot depth: deeper pivots halve the HBM traffic per cell update of the independent
-    // kernel (2 s / T bytes per update); 64 fits the 16-bit strips in 34 KB of LDS
-    uint32_t T = use16 ? 64 : 32;
+    // pivot depth and look-ahead from the tuning table: deeper pivots halve the HBM traffic
+    // per cell update of the independent kernel (2 s / T bytes per update) and the number
+    // of rounds, shallower ones shorten each round's dependent chain (pivot closure and
+    // panels), which wins while rounds are short.  Flags override the table.
+    uint32_t T = 64;
+    bool la_tuned = true;
+    tuned_closure_shape(N, use16 ? (f16 ? SE_TUNE_F16 : SE_TUNE_U16) : SE_TUNE_U32, &T, &la_tuned);
     if (flags & SE_FLAG_PIVOT32) T = 32;
     if (flags & SE_FLAG_PIVOT64) T = 64;
     uint32_t* pivot_buf = static_cast<uint32_t*>(scratch(ctx, kSlotPivot, 2 * 64 * 64 * sizeof(uint32_t)));
@@ -604,15 +608,14 @@ int closure_device_ex(DeviceContext* ctx, const void* d_any, bool d16, uint32_t 
     uint16_t* W16 = static_cast<uint16_t*>(W);
     uint32_t* W32 = static_cast<uint32_t*>(W);
     // look-ahead: the 16-bit paths run the next panel inside the launch of the current
-    // round's remaining tiles (one stream, any size); the 32-bit path overlaps it from a
-    // side stream, which pays once a round's independent work clearly outlasts the panel
-    // kernel (from N = 8192 on).  SE_FLAG_NO_LOOKAHEAD / SE_FLAG_LOOKAHEAD override.
-    const bool no_la = (flags & SE_FLAG_NO_LOOKAHEAD) != 0;
-    const bool la32 = !use16 && !no_la && (N >= 8192 || (flags & SE_FLAG_LOOKAHEAD));
+    // round's remaining tiles (one stream); the 32-bit path overlaps it from a side stream,
+    // which pays once a round's independent work clearly outlasts the panel kernel.
+    // SE_FLAG_NO_LOOKAHEAD / SE_FLAG_LOOKAHEAD override the table.
+    const bool la = (flags & SE_FLAG_NO_LOOKAHEAD) ? false : ((flags & SE_FLAG_LOOKAHEAD) ? true : la_tuned);
+    const bool la32 = !use16 && la;
     if (la32 && !ensure_side_stream(ctx)) return SE_ERR_DEVICE;
     EventTimer tm_side((flags & SE_FLAG_TIME_KERNELS) != 0 && la32, ctx->side);
     if (use16) {
-        const bool la = !no_la;
         if (f16)
             rc = T == 64 ? run_rounds16<64, true>(W16, ldw, n, N, pivot_buf, s, tm, la, &rounds)
                          : run_rounds16<32, true>(W16, ldw, n, N, pivot_buf, s, tm, la, &rounds);

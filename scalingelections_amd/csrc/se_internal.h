@@ -68,6 +68,12 @@ int upload_rows_narrow16(DeviceContext* ctx, hipStream_t s, uint16_t* dev, const
 int download_rows_widen16(DeviceContext* ctx, hipStream_t s, uint32_t* host, size_t host_pitch, const uint16_t* dev,
                           size_t rows, size_t n);
 
+// Launch shapes from the tuning table (tune.cpp).  width: SE_TUNE_F16/U16/U32; N: padded
+// side.  The batch threshold: elections with at least that many candidates share
+// per-round launches.
+void tuned_closure_shape(uint32_t N, uint32_t width, uint32_t* pivot, bool* lookahead);
+uint32_t tuned_batch_rounds_min_c();
+
 // Context of the current HIP device (created on first use).  Returns nullptr and sets
 // the error when no device is usable.
 DeviceContext* current_context();

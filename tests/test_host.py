@@ -300,3 +300,57 @@ def test_ballot_validation_before_device_use():
         from scalingelections_amd import builder
 
         builder.rankings_to_positions([np.array([0, -1])], 2)
+
+
+def test_tuning_table_api():
+    """The launch-shape table: built-in entries cover every width and side, installs and
+    resets round-trip, invalid entries are rejected with a message, and the bucket rule
+    (the smallest measured side >= N decides) merges equal neighbours."""
+    from scalingelections_amd import _lib, autotune
+
+    autotune.reset()
+    base = autotune.current_table()
+    for w in ("f16", "u16", "u32"):
+        assert any(e["width"] == w and e["max_side"] == autotune.ANY_SIDE for e in base), w
+        assert all(e["pivot"] in (32, 64) for e in base)
+    assert autotune.batch_rounds_min() == 129
+    try:
+        mine = [{"width": "u32", "max_side": 512, "pivot": 64, "lookahead": True}]
+        autotune.install(mine, 1025)
+        assert autotune.current_table() == mine and autotune.batch_rounds_min() == 1025
+        L = _lib.lib()
+        bad = (_lib.se_tune_entry * 1)(_lib.se_tune_entry(0, 512, 48, 0))
+        assert L.se_tune_closure(bad, 1) == _lib.SE_ERR_INVALID and "pivot 32 or 64" in _lib.last_error()
+        bad[0] = _lib.se_tune_entry(3, 512, 32, 0)
+        assert L.se_tune_closure(bad, 1) == _lib.SE_ERR_INVALID
+        bad[0] = _lib.se_tune_entry(0, 0, 32, 0)
+        assert L.se_tune_closure(bad, 1) == _lib.SE_ERR_INVALID
+        assert L.se_tune_closure(None, 65) == _lib.SE_ERR_INVALID
+        for th in (1, 128, 1026):
+            assert L.se_tune_batch(th) == _lib.SE_ERR_INVALID and "129..1025" in _lib.last_error()
+        assert autotune.current_table() == mine and autotune.batch_rounds_min() == 1025
+    finally:
+        autotune.reset()
+    assert autotune.current_table() == base and autotune.batch_rounds_min() == 129
+    got = autotune.buckets({1024: (64, True), 256: (32, True), 512: (32, True), 2048: (32, False)}, "u16")
+    assert got == [{"width": "u16", "max_side": 512, "pivot": 32, "lookahead": True},
+                   {"width": "u16", "max_side": 1024, "pivot": 64, "lookahead": True},
+                   {"width": "u16", "max_side": autotune.ANY_SIDE, "pivot": 32, "lookahead": False}]
+    assert autotune.padded_side(1) == 128 and autotune.padded_side(256) == 256 and autotune.padded_side(257) == 384
+
+
+def test_tuning_table_save_load_roundtrip(tmp_path, monkeypatch):
+    from scalingelections_amd import api, autotune
+
+    monkeypatch.setattr(api, "device_info", lambda device=0: {"name": "test"})
+    entries = [{"width": "f16", "max_side": 1024, "pivot": 32, "lookahead": False},
+               {"width": "f16", "max_side": autotune.ANY_SIDE, "pivot": 64, "lookahead": True}]
+    path = str(tmp_path / "tune.json")
+    try:
+        autotune.save(path, entries, 257, {"f16/256": {"T32": 0.1}})
+        autotune.reset()
+        doc = autotune.load(path)
+        assert doc["device"] == {"name": "test"}
+        assert autotune.current_table() == entries and autotune.batch_rounds_min() == 257
+    finally:
+        autotune.reset()
