@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace se {
 
@@ -71,20 +72,20 @@ __device__ __forceinline__ void tally(int32_t& acc, uint32_t x, uint32_t y) {
     }
 }
 
-// One row of 8 cells: the 8 subtractions first, then the 8 dot products, so no dot waits
-// on the instruction just issued (measured 2-3% faster than alternating them).
-template <uint32_t K>
+// One row of NB (8 or 4) cells: the subtractions first, then the dot products, so no dot
+// waits on the instruction just issued (measured 2-3% faster than alternating them).
+template <uint32_t K, int NB = 8>
 __device__ __forceinline__ void tally_row(int32_t* acc, uint32_t x, const uint32_t* y) {
     if constexpr (K == 1) {
 #pragma unroll
-        for (int b = 0; b < 8; ++b) tally<1>(acc[b], x, y[b]);
+        for (int b = 0; b < NB; ++b) tally<1>(acc[b], x, y[b]);
     } else {
-        uint32_t z[8];
+        uint32_t z[NB];
 #pragma unroll
-        for (int b = 0; b < 8; ++b) z[b] = x - y[b];
+        for (int b = 0; b < NB; ++b) z[b] = x - y[b];
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
+        for (int b = 0; b < NB; ++b) {
             if constexpr (K == 4)
                 acc[b] = __builtin_amdgcn_sdot4((int)z[b], 0x01010101, acc[b], false);
             else
@@ -193,6 +194,10 @@ __device__ __forceinline__ void reduce_sums(uint32_t* sums, const StageLane& L, 
 // n candidates: positions pos (row pitch ldp) -> d (pitch ldd).  Lane (ty, tx) owns rows
 // {4 ty + a, 64 + 4 ty + a} and columns {4 tx + b, 64 + 4 tx + b} (a, b < 4): per word a
 // wave reads four 16-byte row chunks (broadcast) and 256 contiguous column bytes.
+// On a diagonal tile (I == J) every lane's bottom-left quadrant (rows 64 + 4 ty + a,
+// columns 4 tx + b) lies below the diagonal: it is not counted, and those cells are
+// written as the complement of the top-right quadrant, d[j][i] = m - d[i][j] (a strict
+// order per voter), which saves a quarter of the tile's counting.
 // PI/PJ: LDS staging of kVW words x kCT candidates; red: 2 x kCT words (may alias PI).
 // With `accumulate` the counts are added to d instead of stored.  Returns the OR of all
 // staged positions (range check of the caller).
@@ -214,22 +219,37 @@ __device__ __forceinline__ uint32_t count_tile(const int32_t* __restrict__ pos, 
     uint32_t s = 0, bits = 0;
     const StageLane SL = stage_lane(ri, rj, false);
 
-    for (uint32_t w0 = 0; w0 < words; w0 += kVW) {
-        __syncthreads();
-        stage_words<K, false>(pos, ldp, vb, ve, w0, I0, J0, SL, PI, PJ, s, bits);
-        __syncthreads();
-        const uint32_t wend = min(kVW, words - w0);
-        for (uint32_t w = 0; w < wend; ++w) {
-            const uint4 i0 = *reinterpret_cast<const uint4*>(&PI[w][4 * ty]);
-            const uint4 i1 = *reinterpret_cast<const uint4*>(&PI[w][64 + 4 * ty]);
-            const uint4 j0 = *reinterpret_cast<const uint4*>(&PJ[w][4 * tx]);
-            const uint4 j1 = *reinterpret_cast<const uint4*>(&PJ[w][64 + 4 * tx]);
-            const uint32_t x[8] = {i0.x, i0.y, i0.z, i0.w, i1.x, i1.y, i1.z, i1.w};
-            const uint32_t y[8] = {j0.x, j0.y, j0.z, j0.w, j1.x, j1.y, j1.z, j1.w};
+    const bool diag = I == J;
+    auto count_words = [&](auto diag_tag) {
+        constexpr bool DIAG = decltype(diag_tag)::value;
+        for (uint32_t w0 = 0; w0 < words; w0 += kVW) {
+            __syncthreads();
+            stage_words<K, false>(pos, ldp, vb, ve, w0, I0, J0, SL, PI, PJ, s, bits);
+            __syncthreads();
+            const uint32_t wend = min(kVW, words - w0);
+            for (uint32_t w = 0; w < wend; ++w) {
+                const uint4 i0 = *reinterpret_cast<const uint4*>(&PI[w][4 * ty]);
+                const uint4 i1 = *reinterpret_cast<const uint4*>(&PI[w][64 + 4 * ty]);
+                const uint4 j0 = *reinterpret_cast<const uint4*>(&PJ[w][4 * tx]);
+                const uint4 j1 = *reinterpret_cast<const uint4*>(&PJ[w][64 + 4 * tx]);
+                const uint32_t x[8] = {i0.x, i0.y, i0.z, i0.w, i1.x, i1.y, i1.z, i1.w};
+                const uint32_t y[8] = {j0.x, j0.y, j0.z, j0.w, j1.x, j1.y, j1.z, j1.w};
 #pragma unroll
-            for (int a = 0; a < 8; ++a) tally_row<K>(acc[a], x[a], y);
+                for (int a = 0; a < 4; ++a) tally_row<K>(acc[a], x[a], y);
+#pragma unroll
+                for (int a = 4; a < 8; ++a) {
+                    if constexpr (DIAG)
+                        tally_row<K, 4>(acc[a] + 4, x[a], y + 4);
+                    else
+                        tally_row<K>(acc[a], x[a], y);
+                }
+            }
         }
-    }
+    };
+    if (diag)
+        count_words(std::true_type{});
+    else
+        count_words(std::false_type{});
     reduce_sums(red, SL, s);  // row sums in red[0, 128), column sums in red[128, 256)
 
     const uint32_t M = words * K, mreal = ve - vb;
@@ -248,12 +268,21 @@ __device__ __forceinline__ uint32_t count_tile(const int32_t* __restrict__ pos, 
 #pragma unroll
         for (int a = 0; a < 8; ++a) {
             const uint32_t i = I0 + (a < 4 ? 4 * ty + a : 64 + 4 * ty + (a - 4));
-            put4(d + (size_t)i * ldd + J0 + 4 * tx, make_uint4(out[a][0], out[a][1], out[a][2], out[a][3]),
-                 accumulate);
+            if (a < 4 || !diag)
+                put4(d + (size_t)i * ldd + J0 + 4 * tx, make_uint4(out[a][0], out[a][1], out[a][2], out[a][3]),
+                     accumulate);
             put4(d + (size_t)i * ldd + J0 + 64 + 4 * tx, make_uint4(out[a][4], out[a][5], out[a][6], out[a][7]),
                  accumulate);
         }
-        if (I != J) {
+        if (diag) {  // bottom-left quadrant: complement of the top-right one
+#pragma unroll
+            for (int b = 4; b < 8; ++b) {
+                const uint32_t j = J0 + 64 + 4 * tx + (b - 4);
+                put4(d + (size_t)j * ldd + I0 + 4 * ty,
+                     make_uint4(mreal - out[0][b], mreal - out[1][b], mreal - out[2][b], mreal - out[3][b]),
+                     accumulate);
+            }
+        } else {
 #pragma unroll
             for (int b = 0; b < 8; ++b) {
                 const uint32_t j = J0 + (b < 4 ? 4 * tx + b : 64 + 4 * tx + (b - 4));
@@ -274,8 +303,11 @@ __device__ __forceinline__ uint32_t count_tile(const int32_t* __restrict__ pos, 
             for (int b = 0; b < 8; ++b) {
                 const uint32_t lj = b < 4 ? 4 * tx + b : 64 + 4 * tx + (b - 4);
                 if (lj >= rj) continue;
-                put1(d + (size_t)(I0 + li) * ldd + J0 + lj, out[a][b], accumulate);
-                if (I != J) put1(d + (size_t)(J0 + lj) * ldd + I0 + li, mreal - out[a][b], accumulate);
+                // diagonal tile: the bottom-left quadrant (a >= 4, b < 4) was not counted;
+                // it is written from the top-right one (a < 4, b >= 4) by complement
+                if (!diag || a < 4 || b >= 4) put1(d + (size_t)(I0 + li) * ldd + J0 + lj, out[a][b], accumulate);
+                if (!diag || (a < 4 && b >= 4))
+                    put1(d + (size_t)(J0 + lj) * ldd + I0 + li, mreal - out[a][b], accumulate);
             }
         }
     }

@@ -2,9 +2,10 @@
 //
 // The host entry points receive and return pageable caller buffers (the drop-in
 // compute_strongest_paths returns a fresh numpy array).  A plain hipMemcpy from pageable
-// memory moves about 10 GB/s; here each matrix goes through two pinned chunks of up to
-// 64 MiB: host worker threads copy rows between the caller's buffer and one chunk while
-// the DMA engine moves the other, so the copy runs at the DMA rate.  The chunks are
+// memory moves about 10 GB/s; here each matrix goes through two pinned chunks (about an
+// eighth of the matrix each, 2..64 MiB): host worker threads copy rows between the
+// caller's buffer and one chunk while the DMA engine moves the other, so the copy runs at
+// the DMA rate.  The chunks are
 // allocated once per device context (hipHostMalloc) and reused.
 #include <hip/hip_runtime.h>
 
@@ -25,6 +26,22 @@ namespace se {
 namespace {
 
 constexpr size_t kChunkBytes = 64u << 20;
+constexpr size_t kTaskBytes = 256u << 10;  // host rows per worker task
+
+// Rows per staged chunk: about an eighth of the matrix, so that the host copies of one
+// chunk overlap the DMA of the other even for small matrices, between 2 MiB and the
+// 64 MiB chunk (row_bytes <= kChunkBytes is checked by the callers).
+size_t rows_per_chunk(size_t rows, size_t row_bytes) {
+    const size_t target = std::min(kChunkBytes, std::max<size_t>(2u << 20, rows * row_bytes / 8));
+    return std::max<size_t>(1, target / row_bytes);
+}
+
+// Rows per worker task when `rows` rows are copied: everything in one task (the calling
+// thread) up to 1 MiB, where waking the pool costs more than it saves.
+size_t rows_per_task(size_t rows, size_t row_bytes) {
+    if (rows * row_bytes <= (1u << 20)) return std::max<size_t>(1, rows);
+    return std::max<size_t>(1, kTaskBytes / std::max<size_t>(1, row_bytes));
+}
 
 // Fixed pool of host threads for row copies.  run(count, fn) calls fn(i) for i < count
 // from the workers and the calling thread and returns when all are done.
@@ -110,7 +127,7 @@ class RowPool {
 // Copies `rows` rows of `row_bytes` between strided host memory and a packed buffer, in
 // parallel blocks of rows.
 void copy_rows(char* dst, size_t dst_pitch, const char* src, size_t src_pitch, size_t rows, size_t row_bytes) {
-    const size_t per = std::max<size_t>(1, (1u << 20) / std::max<size_t>(1, row_bytes));  // about 1 MiB per task
+    const size_t per = rows_per_task(rows, row_bytes);
     const size_t tasks = (rows + per - 1) / per;
     RowPool::get().run(tasks, [&](size_t t) {
         const size_t r0 = t * per, r1 = std::min(rows, r0 + per);
@@ -145,7 +162,7 @@ int upload_rows(DeviceContext* ctx, hipStream_t s, void* dev, const void* host, 
                    ? SE_OK
                    : SE_ERR_COPY;
     }
-    const size_t per = kChunkBytes / row_bytes;
+    const size_t per = rows_per_chunk(rows, row_bytes);
     bool used[2] = {false, false};
     for (size_t r0 = 0, c = 0; r0 < rows; r0 += per, ++c) {
         const size_t nr = std::min(per, rows - r0);
@@ -174,7 +191,7 @@ int download_rows(DeviceContext* ctx, hipStream_t s, void* host, size_t host_pit
             return SE_ERR_COPY;
         return hip_ok(hipStreamSynchronize(s), "CUDA transfers did not complete successfully") ? SE_OK : SE_ERR_COPY;
     }
-    const size_t per = kChunkBytes / row_bytes;
+    const size_t per = rows_per_chunk(rows, row_bytes);
     const size_t chunks = (rows + per - 1) / per;
     auto issue = [&](size_t c) {
         const size_t r0 = c * per, nr = std::min(per, rows - r0);
@@ -206,7 +223,7 @@ int upload_rows_narrow16(DeviceContext* ctx, hipStream_t s, uint16_t* dev, const
     const size_t row_bytes = n * sizeof(uint16_t);
     if (rows == 0 || n == 0) return SE_OK;
     if (row_bytes > kChunkBytes || !ensure_pinned(ctx)) return SE_NARROW_FAILED;
-    const size_t per = kChunkBytes / row_bytes;
+    const size_t per = rows_per_chunk(rows, row_bytes);
     bool used[2] = {false, false};
     std::atomic<uint32_t> high{0};
     for (size_t r0 = 0, c = 0; r0 < rows; r0 += per, ++c) {
@@ -215,9 +232,9 @@ int upload_rows_narrow16(DeviceContext* ctx, hipStream_t s, uint16_t* dev, const
         if (used[b] && !hip_ok(hipEventSynchronize(ctx->pinned_ev[b]), "Failed to copy data from host to device"))
             return SE_ERR_COPY;
         uint16_t* stage = static_cast<uint16_t*>(ctx->pinned[b]);
-        const size_t rows_per_task = std::max<size_t>(1, (1u << 20) / row_bytes);
-        RowPool::get().run((nr + rows_per_task - 1) / rows_per_task, [&](size_t t) {
-            const size_t a0 = t * rows_per_task, a1 = std::min(nr, a0 + rows_per_task);
+        const size_t rpt = rows_per_task(nr, row_bytes);
+        RowPool::get().run((nr + rpt - 1) / rpt, [&](size_t t) {
+            const size_t a0 = t * rpt, a1 = std::min(nr, a0 + rpt);
             uint32_t acc = 0;
             for (size_t r = a0; r < a1; ++r) {
                 const uint32_t* src =
@@ -253,7 +270,7 @@ int download_rows_widen16(DeviceContext* ctx, hipStream_t s, uint32_t* host, siz
         set_error("Failed to copy data from device to host");
         return SE_ERR_COPY;
     }
-    const size_t per = kChunkBytes / row_bytes;
+    const size_t per = rows_per_chunk(rows, row_bytes);
     const size_t chunks = (rows + per - 1) / per;
     auto issue = [&](size_t c) {
         const size_t r0 = c * per, nr = std::min(per, rows - r0);
@@ -270,9 +287,9 @@ int download_rows_widen16(DeviceContext* ctx, hipStream_t s, uint32_t* host, siz
             return SE_ERR_COPY;
         const size_t r0 = c * per, nr = std::min(per, rows - r0);
         const uint16_t* stage = static_cast<const uint16_t*>(ctx->pinned[b]);
-        const size_t rows_per_task = std::max<size_t>(1, (1u << 20) / row_bytes);
-        RowPool::get().run((nr + rows_per_task - 1) / rows_per_task, [&](size_t t) {
-            const size_t a0 = t * rows_per_task, a1 = std::min(nr, a0 + rows_per_task);
+        const size_t rpt = rows_per_task(nr, row_bytes);
+        RowPool::get().run((nr + rpt - 1) / rpt, [&](size_t t) {
+            const size_t a0 = t * rpt, a1 = std::min(nr, a0 + rpt);
             for (size_t r = a0; r < a1; ++r) {
                 uint32_t* dst = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(host) + (r0 + r) * host_pitch);
                 const uint16_t* src = stage + r * n;
