@@ -201,8 +201,30 @@ __device__ __forceinline__ void indep_tile_u16_impl(uint16_t* __restrict__ W, si
     const uint32_t t = threadIdx.x, tx = t & 15, ty = t >> 4;
     __syncthreads();
 
-    // C tile; pivot rows/columns are neither stored nor read here (their cells may be
-    // written concurrently by the next round's panel kernel)
+    // Loads: the strips first, then the C tile.  Memory loads retire in order, so the
+    // strips' LDS stores wait for the strip loads only, and the C loads complete behind the
+    // product A (x) B, which starts from zero (the least value) and is merged into C at the
+    // end -- exact, max being associative.  Cells of the pivot block (K, K) only feed
+    // pivot-row/column cells, which are not stored, so they are not read (zeros instead):
+    // the next round's panel kernel may be committing that block concurrently; pivot
+    // rows/columns of C are neither read nor stored either.
+    constexpr uint32_t kAq = kMacro * (T / 8) / kThreads, kBq = T * 16 / kThreads;
+    static_assert(kMacro * (T / 8) % kThreads == 0 && T * 16 % kThreads == 0, "strip chunks per lane");
+    uint4 sa[kAq], sb[kBq];
+#pragma unroll
+    for (uint32_t i = 0; i < kAq; ++i) {
+        const uint32_t q = t + i * kThreads, row = q / (T / 8), ch = q % (T / 8);
+        sa[i] = make_uint4(0, 0, 0, 0);
+        if (!MASKED || I0 + row - kb >= T)
+            sa[i] = *reinterpret_cast<const uint4*>(W + (size_t)(I0 + row) * ldw + kb + ch * 8);
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < kBq; ++i) {
+        const uint32_t q = t + i * kThreads, row = q >> 4, ch = q & 15;
+        sb[i] = make_uint4(0, 0, 0, 0);
+        if (!MASKED || J0 + ch * 8 - kb >= T)
+            sb[i] = *reinterpret_cast<const uint4*>(W + (size_t)(kb + row) * ldw + J0 + ch * 8);
+    }
     uint4 c[8];
     const uint16_t* cp = W + (size_t)(I0 + ty) * ldw + J0 + tx * 8;
     const bool col_in_pivot = MASKED && J0 + tx * 8 - kb < T;
@@ -216,23 +238,15 @@ __device__ __forceinline__ void indep_tile_u16_impl(uint16_t* __restrict__ W, si
             c[r] = *reinterpret_cast<const uint4*>(cp + (size_t)(16 * r) * ldw);
         }
     }
-
-    // strips; cells of the pivot block (K, K) only feed pivot-row/column cells, which are
-    // not stored, so they are not read (zeros instead): the next round's panel kernel may be
-    // committing that block concurrently
-    for (uint32_t q = t; q < kMacro * (T / 8); q += kThreads) {
-        uint32_t row = q / (T / 8), ch = q % (T / 8);
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (!MASKED || I0 + row - kb >= T)
-            v = *reinterpret_cast<const uint4*>(W + (size_t)(I0 + row) * ldw + kb + ch * 8);
-        *reinterpret_cast<uint4*>(As + row * AP + ch * 8) = v;
+#pragma unroll
+    for (uint32_t i = 0; i < kAq; ++i) {
+        const uint32_t q = t + i * kThreads;
+        *reinterpret_cast<uint4*>(As + (q / (T / 8)) * AP + (q % (T / 8)) * 8) = sa[i];
     }
-    for (uint32_t q = t; q < T * 16; q += kThreads) {
-        uint32_t row = q >> 4, ch = q & 15;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (!MASKED || J0 + ch * 8 - kb >= T)
-            v = *reinterpret_cast<const uint4*>(W + (size_t)(kb + row) * ldw + J0 + ch * 8);
-        *reinterpret_cast<uint4*>(Bs + row * kMacro + ch * 8) = v;
+#pragma unroll
+    for (uint32_t i = 0; i < kBq; ++i) {
+        const uint32_t q = t + i * kThreads;
+        *reinterpret_cast<uint4*>(Bs + (q >> 4) * kMacro + (q & 15) * 8) = sb[i];
     }
     __syncthreads();
 
@@ -240,17 +254,19 @@ __device__ __forceinline__ void indep_tile_u16_impl(uint16_t* __restrict__ W, si
 #pragma unroll
     for (int r = 0; r < 8; ++r)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) acc[r][q] = as_u16x2(word(c[r], q));
+        for (int q = 0; q < 4; ++q) acc[r][q] = as_u16x2(0u);
 
+    // A fragments 4 pivot steps at a time (8 B per row): 16 registers, which keeps the
+    // kernel within 128 VGPRs (4 waves per SIMD) while C is held for the final merge
 #pragma unroll 1
-    for (uint32_t k0 = 0; k0 < T; k0 += 8) {
-        uint4 a4[8];
+    for (uint32_t k0 = 0; k0 < T; k0 += 4) {
+        uint2 a4[8];
 #pragma unroll
         for (int r = 0; r < 8; ++r)
-            a4[r] = *reinterpret_cast<const uint4*>(As + (ty + 16 * r) * AP + k0);
+            a4[r] = *reinterpret_cast<const uint2*>(As + (ty + 16 * r) * AP + k0);
         if constexpr (F16) {
 #pragma unroll
-            for (int kk = 0; kk < 8; kk += 2) {
+            for (int kk = 0; kk < 4; kk += 2) {
                 const uint4 b4 = *reinterpret_cast<const uint4*>(Bs + (k0 + kk) * kMacro + tx * 8);
                 const uint4 b5 = *reinterpret_cast<const uint4*>(Bs + (k0 + kk + 1) * kMacro + tx * 8);
                 const h16x2 b0[4] = {as_h16x2(b4.x), as_h16x2(b4.y), as_h16x2(b4.z), as_h16x2(b4.w)};
@@ -262,7 +278,7 @@ __device__ __forceinline__ void indep_tile_u16_impl(uint16_t* __restrict__ W, si
                     h16x2 m0[2][4], m1[2][4];
 #pragma unroll
                     for (int rr = 0; rr < 2; ++rr) {
-                        const h16x2 a2 = as_h16x2(word(a4[r + rr], kk >> 1));
+                        const h16x2 a2 = as_h16x2((kk >> 1) ? a4[r + rr].y : a4[r + rr].x);
                         const h16x2 a0 = a2.xx, a1 = a2.yy;
 #pragma unroll
                         for (int q = 0; q < 4; ++q) {
@@ -282,7 +298,7 @@ __device__ __forceinline__ void indep_tile_u16_impl(uint16_t* __restrict__ W, si
             }
         } else {
 #pragma unroll
-            for (int kk = 0; kk < 8; ++kk) {
+            for (int kk = 0; kk < 4; ++kk) {
                 const uint4 b4 = *reinterpret_cast<const uint4*>(Bs + (k0 + kk) * kMacro + tx * 8);
                 const u16x2 b[4] = {as_u16x2(b4.x), as_u16x2(b4.y), as_u16x2(b4.z), as_u16x2(b4.w)};
 #pragma unroll
@@ -290,7 +306,7 @@ __device__ __forceinline__ void indep_tile_u16_impl(uint16_t* __restrict__ W, si
                     u16x2 mn[2][4];
 #pragma unroll
                     for (int rr = 0; rr < 2; ++rr) {
-                        const u16x2 a2 = as_u16x2(word(a4[r + rr], kk >> 1));
+                        const u16x2 a2 = as_u16x2((kk >> 1) ? a4[r + rr].y : a4[r + rr].x);
                         const u16x2 av = (kk & 1) ? a2.yy : a2.xx;
 #pragma unroll
                         for (int q = 0; q < 4; ++q) mn[rr][q] = pk_min(av, b[q]);
@@ -303,6 +319,18 @@ __device__ __forceinline__ void indep_tile_u16_impl(uint16_t* __restrict__ W, si
             }
         }
     }
+
+    // merge the product into C
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if constexpr (F16)
+                acc[r][q] = as_u16x2(h_as_u32(
+                    __builtin_elementwise_maximum(as_h16x2(as_u32(acc[r][q])), as_h16x2(word(c[r], q)))));
+            else
+                acc[r][q] = pk_max(acc[r][q], as_u16x2(word(c[r], q)));
+        }
 
     if (MASKED && J0 + tx * 8 - kb < T) return;  // these columns belong to the pivot block
     uint16_t* op = W + (size_t)(I0 + ty) * ldw + J0 + tx * 8;
