@@ -372,7 +372,26 @@ __device__ __forceinline__ void indep_tile64_u16_impl(uint16_t* __restrict__ W, 
     const uint32_t t = threadIdx.x, tx = t & 15, ty = t >> 4;
     __syncthreads();
 
-    uint2 c[4];  // pivot rows/columns neither read nor stored (as in indep_tile_u16)
+    // strips first, then C (merged after the product, as in indep_tile_u16); pivot
+    // rows/columns neither read nor stored
+    constexpr uint32_t kAq = 64 * (T / 8) / kThreads, kBq = T * 8 / kThreads;
+    static_assert(64 * (T / 8) % kThreads == 0 && T * 8 % kThreads == 0, "strip chunks per lane");
+    uint4 sa[kAq], sb[kBq];
+#pragma unroll
+    for (uint32_t i = 0; i < kAq; ++i) {
+        const uint32_t q = t + i * kThreads, row = q / (T / 8), ch = q % (T / 8);
+        sa[i] = make_uint4(0, 0, 0, 0);
+        if (!MASKED || I0 + row - kb >= T)
+            sa[i] = *reinterpret_cast<const uint4*>(W + (size_t)(I0 + row) * ldw + kb + ch * 8);
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < kBq; ++i) {
+        const uint32_t q = t + i * kThreads, row = q >> 3, ch = q & 7;
+        sb[i] = make_uint4(0, 0, 0, 0);
+        if (!MASKED || J0 + ch * 8 - kb >= T)
+            sb[i] = *reinterpret_cast<const uint4*>(W + (size_t)(kb + row) * ldw + J0 + ch * 8);
+    }
+    uint2 c[4];
     const uint16_t* cp = W + (size_t)(I0 + ty) * ldw + J0 + tx * 4;
     const bool col_in_pivot = MASKED && J0 + tx * 4 - kb < T;
 #pragma unroll
@@ -385,28 +404,21 @@ __device__ __forceinline__ void indep_tile64_u16_impl(uint16_t* __restrict__ W, 
             c[r] = *reinterpret_cast<const uint2*>(cp + (size_t)(16 * r) * ldw);
         }
     }
-    for (uint32_t q = t; q < 64 * (T / 8); q += kThreads) {
-        const uint32_t row = q / (T / 8), ch = q % (T / 8);
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (!MASKED || I0 + row - kb >= T)
-            v = *reinterpret_cast<const uint4*>(W + (size_t)(I0 + row) * ldw + kb + ch * 8);
-        *reinterpret_cast<uint4*>(As + row * AP + ch * 8) = v;
+#pragma unroll
+    for (uint32_t i = 0; i < kAq; ++i) {
+        const uint32_t q = t + i * kThreads;
+        *reinterpret_cast<uint4*>(As + (q / (T / 8)) * AP + (q % (T / 8)) * 8) = sa[i];
     }
-    for (uint32_t q = t; q < T * 8; q += kThreads) {
-        const uint32_t row = q >> 3, ch = q & 7;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (!MASKED || J0 + ch * 8 - kb >= T)
-            v = *reinterpret_cast<const uint4*>(W + (size_t)(kb + row) * ldw + J0 + ch * 8);
-        *reinterpret_cast<uint4*>(Bs + row * 64 + ch * 8) = v;
+#pragma unroll
+    for (uint32_t i = 0; i < kBq; ++i) {
+        const uint32_t q = t + i * kThreads;
+        *reinterpret_cast<uint4*>(Bs + (q >> 3) * 64 + (q & 7) * 8) = sb[i];
     }
     __syncthreads();
 
     u16x2 acc[4][2];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        acc[r][0] = as_u16x2(c[r].x);
-        acc[r][1] = as_u16x2(c[r].y);
-    }
+    for (int r = 0; r < 4; ++r) acc[r][0] = acc[r][1] = as_u16x2(0u);
 #pragma unroll 1
     for (uint32_t k0 = 0; k0 < T; k0 += 8) {
         uint4 a4[4];
@@ -437,6 +449,17 @@ __device__ __forceinline__ void indep_tile64_u16_impl(uint16_t* __restrict__ W, 
                     acc[r][1] = pk_max(pk_max(acc[r][1], pk_min(a0, as_u16x2(b4.y))), pk_min(a1, as_u16x2(b5.y)));
                 }
             }
+        }
+    }
+
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        if constexpr (F16) {
+            acc[r][0] = as_u16x2(h_as_u32(__builtin_elementwise_maximum(as_h16x2(as_u32(acc[r][0])), as_h16x2(c[r].x))));
+            acc[r][1] = as_u16x2(h_as_u32(__builtin_elementwise_maximum(as_h16x2(as_u32(acc[r][1])), as_h16x2(c[r].y))));
+        } else {
+            acc[r][0] = pk_max(acc[r][0], as_u16x2(c[r].x));
+            acc[r][1] = pk_max(acc[r][1], as_u16x2(c[r].y));
         }
     }
 
