@@ -56,7 +56,11 @@ extern "C" {
                                     /* values fit the f16-ordered range                   */
 #define SE_FLAG_NO_LOOKAHEAD 256u   /* force off / on the overlap of the next round's      */
 #define SE_FLAG_LOOKAHEAD 512u      /* pivot and panels with the current independent tiles */
-                                    /* (default: the tuning table)                        */
+                                    /* (default: the tuning table); LOOKAHEAD runs the    */
+                                    /* next pivot's strips in a launch of their own       */
+#define SE_FLAG_LOOKAHEAD_FUSED 1024u /* look-ahead in one launch per round: the panel     */
+                                    /* workgroups update the next pivot's strips (16-bit  */
+                                    /* paths with pivot 64; otherwise as LOOKAHEAD)       */
 
 /* Per-call measurements, filled when a non-NULL pointer is passed. */
 typedef struct se_timing {
@@ -184,7 +188,9 @@ typedef struct se_tune_entry {
     uint32_t max_side;   /* applies to zero-padded sides N <= max_side (N: n rounded up to */
                          /* 128); the smallest max_side >= N of the width is used          */
     uint32_t pivot;      /* pivot depth: 32 or 64                                          */
-    uint32_t lookahead;  /* 1: overlap the next pivot's panel with the current round       */
+    uint32_t lookahead;  /* 0: off; 1: the next pivot's panel overlaps the current round,   */
+                         /* its strips in a launch of their own; 2: the same in one launch */
+                         /* per round (16-bit widths with pivot 64 only)                   */
 } se_tune_entry;
 /* Installs a table (entries need not be sorted); count 0 restores the built-in table.
  * Widths without an entry covering N fall back to the built-in table. */

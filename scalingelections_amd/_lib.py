@@ -33,6 +33,7 @@ SE_FLAG_PIVOT64 = 32
 SE_FLAG_NO_F16 = 128
 SE_FLAG_NO_LOOKAHEAD = 256
 SE_FLAG_LOOKAHEAD = 512
+SE_FLAG_LOOKAHEAD_FUSED = 1024
 
 # Every symbol include/scaling_elections_amd.h declares.
 EXPORTED = (

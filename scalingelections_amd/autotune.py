@@ -5,8 +5,9 @@ passes (scaling_elections.cu:895-901).  Here the library picks its launch shapes
 process-wide table (include/scaling_elections_amd.h, se_tune_closure / se_tune_batch):
 
 * per value width (f16-ordered 16-bit, integer 16-bit, 32-bit) and zero-padded side N,
-  the closure's pivot depth (32 or 64) and whether the next pivot's panel overlaps the
-  current round (look-ahead);
+  the closure's pivot depth (32 or 64) and look-ahead mode (0: off; 1: the next pivot's
+  panel overlaps the current round, its strips in a launch of their own; 2: the same in
+  one launch per round);
 * the batch's threshold above which elections share per-round launches instead of being
   closed by one workgroup each.
 
@@ -31,7 +32,15 @@ from . import _lib
 
 WIDTHS = {"f16": _lib.SE_TUNE_F16, "u16": _lib.SE_TUNE_U16, "u32": _lib.SE_TUNE_U32}
 WIDTH_FLAGS = {"f16": 0, "u16": _lib.SE_FLAG_NO_F16, "u32": _lib.SE_FLAG_FORCE_U32}
-CANDIDATES = ((32, False), (32, True), (64, False), (64, True))
+# (pivot, look-ahead mode): 0 off, 1 strips in a launch of their own, 2 one launch per
+# round (16-bit widths, pivot 64)
+CANDIDATES = ((32, 0), (32, 1), (64, 0), (64, 1), (64, 2))
+LA_FLAGS = {0: _lib.SE_FLAG_NO_LOOKAHEAD, 1: _lib.SE_FLAG_LOOKAHEAD, 2: _lib.SE_FLAG_LOOKAHEAD_FUSED}
+LA_NAMES = {0: "", 1: "+LA", 2: "+LA1"}
+
+
+def candidates(width: str):
+    return tuple(c for c in CANDIDATES if not (c[1] == 2 and width == "u32"))
 DEFAULT_SIZES = (256, 512, 1024, 1536, 2048, 3072, 4096, 6144, 8192, 12288, 16384)
 BATCH_THRESHOLDS = (129, 193, 257, 385, 513, 1025)
 ANY_SIDE = 0xFFFFFFFF
@@ -51,7 +60,7 @@ def current_table() -> list[dict]:
     _lib.check(L.se_tune_closure_get(buf, _lib.SE_TUNE_MAX_ENTRIES, ctypes.byref(count)))
     names = {v: k for k, v in WIDTHS.items()}
     return [{"width": names[e.width], "max_side": int(e.max_side), "pivot": int(e.pivot),
-             "lookahead": bool(e.lookahead)} for e in buf[:count.value]]
+             "lookahead": int(e.lookahead)} for e in buf[:count.value]]
 
 
 def install(entries: list[dict] | None = None, batch_rounds_min: int | None = None) -> None:
@@ -61,8 +70,7 @@ def install(entries: list[dict] | None = None, batch_rounds_min: int | None = No
     entries = entries or []
     arr = (_lib.se_tune_entry * max(1, len(entries)))()
     for i, e in enumerate(entries):
-        arr[i] = _lib.se_tune_entry(WIDTHS[e["width"]], int(e["max_side"]), int(e["pivot"]),
-                                    1 if e["lookahead"] else 0)
+        arr[i] = _lib.se_tune_entry(WIDTHS[e["width"]], int(e["max_side"]), int(e["pivot"]), int(e["lookahead"]))
     _lib.check(L.se_tune_closure(arr, len(entries)))
     if batch_rounds_min is not None:
         _lib.check(L.se_tune_batch(int(batch_rounds_min)))
@@ -94,7 +102,7 @@ def load(path: str) -> dict:
     return doc
 
 
-def buckets(best: dict[int, tuple[int, bool]], width: str) -> list[dict]:
+def buckets(best: dict[int, tuple[int, int]], width: str) -> list[dict]:
     """Table entries of one width from {padded side: (pivot, lookahead)}: each measured side
     bounds a bucket (sides between two measured ones take the larger one's choice), the
     largest extends to every side, and neighbours with the same choice merge."""
@@ -137,16 +145,16 @@ def tune_closure(sizes=DEFAULT_SIZES, widths=("f16", "u16", "u32"), min_seconds:
     entries: list[dict] = []
     meas: dict = {}
     for width in widths:
-        best: dict[int, tuple[int, bool]] = {}
+        best: dict[int, tuple[int, int]] = {}
         for n in sizes:
             d = synth.synthesized_preferences(int(n), seed=3)
             dd = devmem.DeviceBuffer.from_array(d)
             pp = devmem.DeviceBuffer(int(n) * int(n) * 4)
             times = {}
             ref = None
-            for pivot, la in CANDIDATES:
+            for pivot, la in candidates(width):
                 flags = WIDTH_FLAGS[width] | (_lib.SE_FLAG_PIVOT32 if pivot == 32 else _lib.SE_FLAG_PIVOT64) \
-                    | (_lib.SE_FLAG_LOOKAHEAD if la else _lib.SE_FLAG_NO_LOOKAHEAD)
+                    | LA_FLAGS[la]
                 times[(pivot, la)] = _time_closure(L, dd, pp, int(n), flags, min_seconds, max_reps)
                 if n <= verify_max_n:
                     out = pp.download(np.empty((int(n), int(n)), np.uint32))
@@ -157,11 +165,11 @@ def tune_closure(sizes=DEFAULT_SIZES, widths=("f16", "u16", "u32"), min_seconds:
                                            f"pivot={pivot} lookahead={la}")
             choice = min(times, key=times.get)
             best[padded_side(n)] = choice
-            meas[f"{width}/{n}"] = {f"T{p}{'+LA' if a else ''}": round(t * 1e3, 4) for (p, a), t in times.items()}
+            meas[f"{width}/{n}"] = {f"T{p}{LA_NAMES[a]}": round(t * 1e3, 4) for (p, a), t in times.items()}
             if log:
-                log(f"{width:4s} n={n:6d}: " + "  ".join(f"T{p}{'+LA' if a else '   '} {t * 1e3:9.3f} ms"
+                log(f"{width:4s} n={n:6d}: " + "  ".join(f"T{p}{LA_NAMES[a]:4s} {t * 1e3:9.3f} ms"
                                                           for (p, a), t in sorted(times.items()))
-                    + f"  -> T{choice[0]}{'+LA' if choice[1] else ''}")
+                    + f"  -> T{choice[0]}{LA_NAMES[choice[1]]}")
             dd.close()
             pp.close()
         entries += buckets(best, width)

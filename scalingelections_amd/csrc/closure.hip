@@ -280,12 +280,17 @@ __global__ __launch_bounds__(256) void strip64_u16_kernel(uint16_t* __restrict__
 // row-major over the full nb x nb grid (strip tiles exit), so tile (bi, bj) lands on XCD
 // bj mod 8 when nb is a multiple of 8: each XCD's L2 serves the B strips of its columns.
 // With a grid of npanel and nothing else the kernel is a plain panel launch.
+//
+// strips_here = 1 (T = 64): no strip launch ran.  The panel workgroups first give their
+// row and column chunks (the next pivot's T rows and columns) this round's update (see
+// panel_u16), and the tiles of macro row/column mnext run here too, with the next pivot's
+// rows and columns masked (kx): every cell still has exactly one writer per launch.
 template <uint32_t T, bool F16>
 __global__ __launch_bounds__(256) void panel_indep_u16_kernel(uint16_t* __restrict__ W, size_t ldw, uint32_t kb,
                                                               uint32_t mnext, uint32_t nb, uint32_t npanel,
                                                               uint32_t kb_next, uint32_t* __restrict__ pivot_out,
                                                               const uint32_t* __restrict__ pivot_prev,
-                                                              uint32_t kb_prev) {
+                                                              uint32_t kb_prev, uint32_t strips_here) {
     static_assert(panel_u16_lds_elems<T>() <= indep_u16_lds_elems<T>(), "panel LDS must fit the tile LDS");
     __shared__ __align__(16) uint16_t lds[indep_u16_lds_elems<T>()];
     if (blockIdx.x < npanel) {
@@ -293,12 +298,17 @@ __global__ __launch_bounds__(256) void panel_indep_u16_kernel(uint16_t* __restri
         // the panel is the round's critical path: its waves issue ahead of the tiles' waves
         // sharing their SIMDs (the tiles fill the issue slots the panel's barriers leave)
         __builtin_amdgcn_s_setprio(3);
-        panel_u16<T, F16>(W, ldw, kb_next, blockIdx.x, pivot_out, pivot_prev, kb_prev, lds);
+        panel_u16<T, F16>(W, ldw, kb_next, blockIdx.x, pivot_out, pivot_prev, kb_prev, lds,
+                          strips_here ? kb : kNoRange);
         return;
     }
     const uint32_t t = blockIdx.x - npanel;
     const uint32_t bi = t / nb, bj = t % nb;
-    if (bi == mnext || bj == mnext) return;  // strip tiles ran in the strip launch
+    if (bi == mnext || bj == mnext) {
+        if (!strips_here) return;  // strip tiles ran in the strip launch
+        indep_tile_u16<T, F16>(W, ldw, kb, bi * kMacro, bj * kMacro, lds, kb_next);
+        return;
+    }
     indep_tile_u16<T, F16>(W, ldw, kb, bi * kMacro, bj * kMacro, lds);
 }
 
@@ -453,13 +463,16 @@ int run_rounds(DeviceContext* ctx, E* W, size_t ldw, uint32_t n, uint32_t N, uin
 }
 
 
-// Rounds of the 16-bit closure.  With look-ahead, round r is two launches: the strip tiles
-// of the next pivot (macro row/column mnext), then one launch holding the next round's
-// panel and the rest of round r's independent tiles (panel_indep_u16_kernel).  Without
-// look-ahead: all independent tiles, then the next panel.
+// Rounds of the 16-bit closure, by look-ahead mode:
+//   kLaStrips: round r is two launches, the strip tiles of the next pivot (macro
+//     row/column mnext), then one launch holding the next round's panel and the rest of
+//     round r's independent tiles (panel_indep_u16_kernel);
+//   kLaFused (T = 64): one launch per round, the panel workgroups giving the next pivot's
+//     strips round r's update themselves (no strip launch);
+//   kLaOff: all independent tiles, then the next panel.
 template <uint32_t T, bool F16>
 int run_rounds16(uint16_t* W, size_t ldw, uint32_t n, uint32_t N, uint32_t* pivot_buf, hipStream_t s,
-                 EventTimer& tm, bool lookahead, uint32_t* rounds_out) {
+                 EventTimer& tm, uint32_t la_mode, uint32_t* rounds_out) {
     const uint32_t nb = N / kMacro;
     const uint32_t rounds = (n + T - 1) / T;
     auto pivot_of = [&](uint32_t r) { return pivot_buf + (r & 1) * T * T; };
@@ -469,7 +482,8 @@ int run_rounds16(uint16_t* W, size_t ldw, uint32_t n, uint32_t N, uint32_t* pivo
     }
     const uint32_t npanel = (nb + 7) / 8 * 8;
     tm.mark(kKindPanel);
-    panel_indep_u16_kernel<T, F16><<<npanel, kThreads, 0, s>>>(W, ldw, 0, 0, nb, npanel, 0, pivot_of(0), nullptr, 0);
+    panel_indep_u16_kernel<T, F16><<<npanel, kThreads, 0, s>>>(W, ldw, 0, 0, nb, npanel, 0, pivot_of(0), nullptr, 0,
+                                                                0u);
     tm.mark(kKindPanel);
     for (uint32_t r = 0; r < rounds; ++r) {
         if (g_interrupted.load(std::memory_order_relaxed)) {
@@ -482,20 +496,26 @@ int run_rounds16(uint16_t* W, size_t ldw, uint32_t n, uint32_t N, uint32_t* pivo
             tm.mark(kKindIndep);
             launch_indep<uint16_t, T, F16>(W, ldw, kb, 0, 0, nb, s);
             tm.mark(kKindIndep);
-        } else if (!lookahead) {
+        } else if (la_mode == kLaOff) {
             tm.mark(kKindIndep);
             launch_indep<uint16_t, T, F16>(W, ldw, kb, 0, 0, nb, s);
             tm.mark(kKindIndep);
             tm.mark(kKindPanel);
-            panel_indep_u16_kernel<T, F16>
-                <<<npanel, kThreads, 0, s>>>(W, ldw, kb, 0, nb, npanel, kb + T, pivot_of(r + 1), pivot_of(r), kb);
+            panel_indep_u16_kernel<T, F16><<<npanel, kThreads, 0, s>>>(W, ldw, kb, 0, nb, npanel, kb + T,
+                                                                        pivot_of(r + 1), pivot_of(r), kb, 0u);
             tm.mark(kKindPanel);
+        } else if (T == 64 && la_mode == kLaFused) {
+            const uint32_t mnext = (kb + T) / kMacro;
+            tm.mark(kKindIndep);
+            panel_indep_u16_kernel<T, F16><<<npanel + nb * nb, kThreads, 0, s>>>(
+                W, ldw, kb, mnext, nb, npanel, kb + T, pivot_of(r + 1), pivot_of(r), kb, 1u);
+            tm.mark(kKindIndep);
         } else {
             const uint32_t mnext = (kb + T) / kMacro;
             tm.mark(kKindIndep);
             strip64_u16_kernel<T, F16><<<4 * (2 * nb - 1), kThreads, 0, s>>>(W, ldw, kb, mnext, nb);
             panel_indep_u16_kernel<T, F16><<<npanel + nb * nb, kThreads, 0, s>>>(
-                W, ldw, kb, mnext, nb, npanel, kb + T, pivot_of(r + 1), pivot_of(r), kb);
+                W, ldw, kb, mnext, nb, npanel, kb + T, pivot_of(r + 1), pivot_of(r), kb, 0u);
             tm.mark(kKindIndep);
         }
         SE_CHECK_LAUNCH("closure round launch failed");
@@ -596,9 +616,8 @@ int closure_device_ex(DeviceContext* ctx, const void* d_any, bool d16, uint32_t 
     // per cell update of the independent kernel (2 s / T bytes per update) and the number
     // of rounds, shallower ones shorten each round's dependent chain (pivot closure and
     // panels), which wins while rounds are short.  Flags override the table.
-    uint32_t T = 64;
-    bool la_tuned = true;
-    tuned_closure_shape(N, use16 ? (f16 ? SE_TUNE_F16 : SE_TUNE_U16) : SE_TUNE_U32, &T, &la_tuned);
+    uint32_t T = 64, la_mode = kLaStrips;
+    tuned_closure_shape(N, use16 ? (f16 ? SE_TUNE_F16 : SE_TUNE_U16) : SE_TUNE_U32, &T, &la_mode);
     if (flags & SE_FLAG_PIVOT32) T = 32;
     if (flags & SE_FLAG_PIVOT64) T = 64;
     uint32_t* pivot_buf = static_cast<uint32_t*>(scratch(ctx, kSlotPivot, 2 * 64 * 64 * sizeof(uint32_t)));
@@ -611,17 +630,23 @@ int closure_device_ex(DeviceContext* ctx, const void* d_any, bool d16, uint32_t 
     // round's remaining tiles (one stream); the 32-bit path overlaps it from a side stream,
     // which pays once a round's independent work clearly outlasts the panel kernel.
     // SE_FLAG_NO_LOOKAHEAD / SE_FLAG_LOOKAHEAD override the table.
-    const bool la = (flags & SE_FLAG_NO_LOOKAHEAD) ? false : ((flags & SE_FLAG_LOOKAHEAD) ? true : la_tuned);
-    const bool la32 = !use16 && la;
+    if (flags & SE_FLAG_NO_LOOKAHEAD)
+        la_mode = kLaOff;
+    else if (flags & SE_FLAG_LOOKAHEAD_FUSED)
+        la_mode = kLaFused;
+    else if (flags & SE_FLAG_LOOKAHEAD)
+        la_mode = kLaStrips;
+    if (la_mode == kLaFused && (!use16 || T != 64)) la_mode = kLaStrips;  // 16-bit, pivot 64 only
+    const bool la32 = !use16 && la_mode != kLaOff;
     if (la32 && !ensure_side_stream(ctx)) return SE_ERR_DEVICE;
     EventTimer tm_side((flags & SE_FLAG_TIME_KERNELS) != 0 && la32, ctx->side);
     if (use16) {
         if (f16)
-            rc = T == 64 ? run_rounds16<64, true>(W16, ldw, n, N, pivot_buf, s, tm, la, &rounds)
-                         : run_rounds16<32, true>(W16, ldw, n, N, pivot_buf, s, tm, la, &rounds);
+            rc = T == 64 ? run_rounds16<64, true>(W16, ldw, n, N, pivot_buf, s, tm, la_mode, &rounds)
+                         : run_rounds16<32, true>(W16, ldw, n, N, pivot_buf, s, tm, la_mode, &rounds);
         else
-            rc = T == 64 ? run_rounds16<64, false>(W16, ldw, n, N, pivot_buf, s, tm, la, &rounds)
-                         : run_rounds16<32, false>(W16, ldw, n, N, pivot_buf, s, tm, la, &rounds);
+            rc = T == 64 ? run_rounds16<64, false>(W16, ldw, n, N, pivot_buf, s, tm, la_mode, &rounds)
+                         : run_rounds16<32, false>(W16, ldw, n, N, pivot_buf, s, tm, la_mode, &rounds);
     } else {
         rc = T == 64 ? run_rounds<uint32_t, 64>(ctx, W32, ldw, n, N, pivot_buf, s, tm, tm_side, la32, &rounds)
                      : run_rounds<uint32_t, 32>(ctx, W32, ldw, n, N, pivot_buf, s, tm, tm_side, la32, &rounds);

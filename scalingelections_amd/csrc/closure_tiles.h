@@ -191,10 +191,15 @@ constexpr uint32_t indep_u16_lds_elems() { return kMacro * (T + 8) + T * kMacro;
 // unsigned order; IEEE-754-2019 minimum/maximum return one operand unchanged, so two pivot
 // steps fold into 2 v_pk_minimum3_f16 + 1 v_pk_maximum3_f16 (0.75 VALU per update).
 // MASKED = false: the tile holds no pivot row or column (almost every tile), so loads and
-// stores need no per-row checks.
+// stores need no per-row checks.  kx (kNoRange: none) names a second block of T rows and
+// columns of C that is neither read nor stored: the next pivot's rows and columns when the
+// panel workgroups of the same launch update them (the strips are still read in full).
+constexpr uint32_t kNoRange = 0xFFFFFFFFu;
+
 template <uint32_t T, bool F16, bool MASKED>
 __device__ __forceinline__ void indep_tile_u16_impl(uint16_t* __restrict__ W, size_t ldw, uint32_t kb,
-                                                    uint32_t I0, uint32_t J0, uint16_t* __restrict__ lds) {
+                                                    uint32_t I0, uint32_t J0, uint16_t* __restrict__ lds,
+                                                    uint32_t kx = kNoRange) {
     constexpr uint32_t AP = T + 8;  // A strip pitch (u16), keeps 16-byte alignment
     uint16_t* As = lds;
     uint16_t* Bs = lds + kMacro * AP;
@@ -225,14 +230,16 @@ __device__ __forceinline__ void indep_tile_u16_impl(uint16_t* __restrict__ W, si
         if (!MASKED || J0 + ch * 8 - kb >= T)
             sb[i] = *reinterpret_cast<const uint4*>(W + (size_t)(kb + row) * ldw + J0 + ch * 8);
     }
+    // C rows/columns that are neither read nor stored: this round's pivot block and kx
+    auto skipped = [&](uint32_t x) { return x - kb < T || (kx != kNoRange && x - kx < T); };
     uint4 c[8];
     const uint16_t* cp = W + (size_t)(I0 + ty) * ldw + J0 + tx * 8;
-    const bool col_in_pivot = MASKED && J0 + tx * 8 - kb < T;
+    const bool col_in_pivot = MASKED && skipped(J0 + tx * 8);
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
         if constexpr (MASKED) {
             c[r] = make_uint4(0, 0, 0, 0);
-            if (!col_in_pivot && I0 + ty + 16 * r - kb >= T)
+            if (!col_in_pivot && !skipped(I0 + ty + 16 * r))
                 c[r] = *reinterpret_cast<const uint4*>(cp + (size_t)(16 * r) * ldw);
         } else {
             c[r] = *reinterpret_cast<const uint4*>(cp + (size_t)(16 * r) * ldw);
@@ -332,11 +339,11 @@ __device__ __forceinline__ void indep_tile_u16_impl(uint16_t* __restrict__ W, si
                 acc[r][q] = pk_max(acc[r][q], as_u16x2(word(c[r], q)));
         }
 
-    if (MASKED && J0 + tx * 8 - kb < T) return;  // these columns belong to the pivot block
+    if (col_in_pivot) return;  // these columns belong to a pivot block
     uint16_t* op = W + (size_t)(I0 + ty) * ldw + J0 + tx * 8;
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
-        if (MASKED && I0 + ty + 16 * r - kb < T) continue;  // pivot rows are final already
+        if (MASKED && skipped(I0 + ty + 16 * r)) continue;  // pivot rows: final, or the panel's
         uint4 v;
         v.x = as_u32(acc[r][0]);
         v.y = as_u32(acc[r][1]);
@@ -348,12 +355,14 @@ __device__ __forceinline__ void indep_tile_u16_impl(uint16_t* __restrict__ W, si
 
 template <uint32_t T, bool F16 = false>
 __device__ __forceinline__ void indep_tile_u16(uint16_t* __restrict__ W, size_t ldw, uint32_t kb, uint32_t I0,
-                                               uint32_t J0, uint16_t* __restrict__ lds) {
-    const bool clean = (I0 + kMacro <= kb || I0 >= kb + T) && (J0 + kMacro <= kb || J0 >= kb + T);
-    if (clean)
+                                               uint32_t J0, uint16_t* __restrict__ lds, uint32_t kx = kNoRange) {
+    auto apart = [&](uint32_t k) {
+        return (I0 + kMacro <= k || I0 >= k + T) && (J0 + kMacro <= k || J0 >= k + T);
+    };
+    if (apart(kb) && (kx == kNoRange || apart(kx)))
         indep_tile_u16_impl<T, F16, false>(W, ldw, kb, I0, J0, lds);
     else
-        indep_tile_u16_impl<T, F16, true>(W, ldw, kb, I0, J0, lds);
+        indep_tile_u16_impl<T, F16, true>(W, ldw, kb, I0, J0, lds, kx);
 }
 
 // Quarter tiles (64 x 64) for the strip launch, which has only 2 nb - 1 macro tiles: four
@@ -363,9 +372,11 @@ __device__ __forceinline__ void indep_tile_u16(uint16_t* __restrict__ W, size_t 
 template <uint32_t T>
 constexpr uint32_t indep64_u16_lds_elems() { return 64 * (T + 8) + T * 64; }
 
+// out_lds != nullptr: the updated quarter goes to LDS (pitch 64, every cell) instead of W.
 template <uint32_t T, bool F16, bool MASKED>
 __device__ __forceinline__ void indep_tile64_u16_impl(uint16_t* __restrict__ W, size_t ldw, uint32_t kb,
-                                                      uint32_t I0, uint32_t J0, uint16_t* __restrict__ lds) {
+                                                      uint32_t I0, uint32_t J0, uint16_t* __restrict__ lds,
+                                                      uint16_t* __restrict__ out_lds) {
     constexpr uint32_t AP = T + 8;
     uint16_t* As = lds;
     uint16_t* Bs = lds + 64 * AP;
@@ -463,6 +474,13 @@ __device__ __forceinline__ void indep_tile64_u16_impl(uint16_t* __restrict__ W, 
         }
     }
 
+    if (out_lds != nullptr) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            *reinterpret_cast<uint2*>(out_lds + (ty + 16 * r) * 64 + tx * 4) =
+                make_uint2(as_u32(acc[r][0]), as_u32(acc[r][1]));
+        return;
+    }
     if (MASKED && J0 + tx * 4 - kb < T) return;
     uint16_t* op = W + (size_t)(I0 + ty) * ldw + J0 + tx * 4;
 #pragma unroll
@@ -474,12 +492,13 @@ __device__ __forceinline__ void indep_tile64_u16_impl(uint16_t* __restrict__ W, 
 
 template <uint32_t T, bool F16>
 __device__ __forceinline__ void indep_tile64_u16(uint16_t* __restrict__ W, size_t ldw, uint32_t kb, uint32_t I0,
-                                                 uint32_t J0, uint16_t* __restrict__ lds) {
+                                                 uint32_t J0, uint16_t* __restrict__ lds,
+                                                 uint16_t* __restrict__ out_lds = nullptr) {
     const bool clean = (I0 + 64 <= kb || I0 >= kb + T) && (J0 + 64 <= kb || J0 >= kb + T);
     if (clean)
-        indep_tile64_u16_impl<T, F16, false>(W, ldw, kb, I0, J0, lds);
+        indep_tile64_u16_impl<T, F16, false>(W, ldw, kb, I0, J0, lds, out_lds);
     else
-        indep_tile64_u16_impl<T, F16, true>(W, ldw, kb, I0, J0, lds);
+        indep_tile64_u16_impl<T, F16, true>(W, ldw, kb, I0, J0, lds, out_lds);
 }
 
 // Independent tiles, 32-bit.  Lane (ty, tx) owns rows ty + 16 r (r < 8) and columns
@@ -737,10 +756,19 @@ __device__ __forceinline__ void panel_cols_u16(uint16_t* __restrict__ W, size_t 
 // `chunk`.  Both chunks are read from W before the pivot closes (neither product writes a
 // cell the other reads: they meet only in the pivot block, which neither stores), so their
 // loads overlap the pivot closure.  Starts with a barrier.
+//
+// kr != kNoRange (T = 64 only): this panel's strips have not yet received the update of
+// the round whose pivot block starts at kr (strips final).  The workgroup applies it
+// first, with the 64 x 64 quarter-tile code: to the two quarters of its row chunk and the
+// two of its column chunk (written to W; the pivot block quarter is skipped) and to the
+// pivot block itself, into LDS only -- every panel workgroup needs it, and the block in W
+// is replaced by the closed pivot one launch later.  Cells in kr's rows or columns are
+// final and stay untouched (the quarter code masks them), so the block at kr, which
+// workgroup 0 may be committing meanwhile, is never read.
 template <uint32_t T, bool F16>
 __device__ __forceinline__ void panel_u16(uint16_t* __restrict__ W, size_t ldw, uint32_t kb, uint32_t chunk,
                                           uint32_t* __restrict__ pivot_out, const uint32_t* __restrict__ pivot_prev,
-                                          uint32_t kb_prev, uint16_t* lds) {
+                                          uint32_t kb_prev, uint16_t* lds, uint32_t kr = kNoRange) {
     constexpr uint32_t NQ = T * 16 / kThreads, SP = 136;
     uint16_t* P16 = lds;
     uint16_t* S = lds + T * (T + 2);
@@ -751,6 +779,27 @@ __device__ __forceinline__ void panel_u16(uint16_t* __restrict__ W, size_t ldw, 
     if (chunk == 0 && pivot_prev != nullptr) {
         for (uint32_t q = t; q < T * T; q += kThreads)
             W[(size_t)(kb_prev + q / T) * ldw + kb_prev + q % T] = (uint16_t)pivot_prev[q];
+    }
+    // the pivot block to close: W, or its updated copy in LDS
+    const uint16_t* pv = W;
+    size_t pv_ld = ldw;
+    uint32_t pv_k = kb;
+    if constexpr (T == 64) {
+        if (kr != kNoRange) {
+            uint16_t* pn = lds + indep64_u16_lds_elems<T>();  // beyond the quarter code's LDS
+#pragma unroll 1
+            for (uint32_t q = 0; q < 2; ++q) {
+                const uint32_t j0 = c0 + 64 * q;
+                if (j0 == kb) continue;  // the pivot block: below, into LDS
+                indep_tile64_u16<T, F16>(W, ldw, kr, kb, j0, lds);  // row chunk quarter
+                indep_tile64_u16<T, F16>(W, ldw, kr, j0, kb, lds);  // column chunk quarter
+            }
+            indep_tile64_u16<T, F16>(W, ldw, kr, kb, kb, lds, pn);
+            __syncthreads();  // the quarters' stores and pn are visible to the whole workgroup
+            pv = pn;
+            pv_ld = 64;
+            pv_k = 0;
+        }
     }
     // both chunks into registers; thread q of the column gather takes (k = q % T, rows
     // c0 + 8 g .. 8 g + 7 with g = q / T) and packs them for the transposed layout
@@ -790,7 +839,7 @@ __device__ __forceinline__ void panel_u16(uint16_t* __restrict__ W, size_t ldw, 
         col2 = gather(2);
         col3 = gather(3);
     }
-    close_pivot_u16<T>(W, ldw, kb, P16, rowbuf, colbuf);  // ends with a barrier
+    close_pivot_u16<T>(pv, pv_ld, pv_k, P16, rowbuf, colbuf);  // ends with a barrier
     if (chunk == 0)
         for (uint32_t q = t; q < T * T; q += kThreads) pivot_out[q] = P16[(q / T) * (T + 2) + q % T];
     put_row(0, row0);

@@ -68,10 +68,15 @@ int upload_rows_narrow16(DeviceContext* ctx, hipStream_t s, uint16_t* dev, const
 int download_rows_widen16(DeviceContext* ctx, hipStream_t s, uint32_t* host, size_t host_pitch, const uint16_t* dev,
                           size_t rows, size_t n);
 
+// Look-ahead modes of the closure schedule (the tuning table's lookahead values):
+// off; the next pivot's strips in a launch of their own; those strips updated by the
+// panel workgroups inside the round's one launch (16-bit, pivot 64).
+constexpr uint32_t kLaOff = 0, kLaStrips = 1, kLaFused = 2;
+
 // Launch shapes from the tuning table (tune.cpp).  width: SE_TUNE_F16/U16/U32; N: padded
 // side.  The batch threshold: elections with at least that many candidates share
 // per-round launches.
-void tuned_closure_shape(uint32_t N, uint32_t width, uint32_t* pivot, bool* lookahead);
+void tuned_closure_shape(uint32_t N, uint32_t width, uint32_t* pivot, uint32_t* lookahead);
 uint32_t tuned_batch_rounds_min_c();
 
 // Context of the current HIP device (created on first use).  Returns nullptr and sets

@@ -18,15 +18,17 @@ namespace se {
 namespace {
 
 // Measured on MI355X by the autotuner (sides 256..16384, synthesized preferences, best of
-// pivot 32/64 x look-ahead on/off): on the 16-bit paths pivot 32 wins while rounds are
-// short (N <= 2048: 1024 f16 0.50 vs 0.56 ms), pivot 64 above (4096 f16: 2.84 vs 3.25 ms),
-// always with the in-launch look-ahead; the 32-bit path prefers pivot 32 up to 1024 and
-// pivot 64 above (8192: 31.2 vs 37.2 ms), and its side-stream look-ahead only pays from
-// 16384 on (217 vs 222 ms).
+// every pivot depth and look-ahead mode): on the 16-bit paths pivot 32 wins while rounds
+// are short (N <= 2048: 1024 f16 0.50 vs 0.56 ms), pivot 64 above (4096 f16: 2.64 vs
+// 3.03 ms); the one-launch-per-round look-ahead wins from 6144 on (8192 f16: 13.56 vs
+// 14.24 ms), where a round's tiles outlast the panel workgroups' extra strip update, and
+// loses below (4096: 2.83 vs 2.64 ms).  The 32-bit path prefers pivot 32 up to 1024 and
+// pivot 64 above (8192: 31.2 vs 37.2 ms); its side-stream look-ahead only pays from 16384
+// on (217 vs 222 ms).
 const std::vector<se_tune_entry>& builtin_table() {
     static const std::vector<se_tune_entry> t = {
-        {SE_TUNE_F16, 2048, 32, 1},       {SE_TUNE_F16, 0xFFFFFFFFu, 64, 1},
-        {SE_TUNE_U16, 2048, 32, 1},       {SE_TUNE_U16, 0xFFFFFFFFu, 64, 1},
+        {SE_TUNE_F16, 2048, 32, 1},       {SE_TUNE_F16, 4096, 64, 1},       {SE_TUNE_F16, 0xFFFFFFFFu, 64, 2},
+        {SE_TUNE_U16, 2048, 32, 1},       {SE_TUNE_U16, 4096, 64, 1},       {SE_TUNE_U16, 0xFFFFFFFFu, 64, 2},
         {SE_TUNE_U32, 1024, 32, 0},       {SE_TUNE_U32, 12288, 64, 0},
         {SE_TUNE_U32, 0xFFFFFFFFu, 64, 1},
     };
@@ -54,14 +56,14 @@ bool lookup(const std::vector<se_tune_entry>& t, uint32_t N, uint32_t width, se_
 
 }  // namespace
 
-void tuned_closure_shape(uint32_t N, uint32_t width, uint32_t* pivot, bool* lookahead) {
-    se_tune_entry e{width, 0, 64, 1};
+void tuned_closure_shape(uint32_t N, uint32_t width, uint32_t* pivot, uint32_t* lookahead) {
+    se_tune_entry e{width, 0, 64, kLaStrips};
     {
         std::lock_guard<std::mutex> g(g_tune_mutex);
         if (g_table.empty() || !lookup(g_table, N, width, &e)) lookup(builtin_table(), N, width, &e);
     }
     *pivot = e.pivot;
-    *lookahead = e.lookahead != 0;
+    *lookahead = e.lookahead;
 }
 
 uint32_t tuned_batch_rounds_min_c() {
@@ -83,9 +85,10 @@ int se_tune_closure(const se_tune_entry* entries, uint32_t count) {
     }
     for (uint32_t i = 0; i < count; ++i) {
         const se_tune_entry& e = entries[i];
-        if (e.width > SE_TUNE_U32 || (e.pivot != 32 && e.pivot != 64) || e.lookahead > 1 || e.max_side == 0) {
-            set_error("tuning table: width must be SE_TUNE_F16/U16/U32, pivot 32 or 64, lookahead 0 or 1, "
-                      "max_side > 0");
+        if (e.width > SE_TUNE_U32 || (e.pivot != 32 && e.pivot != 64) || e.lookahead > kLaFused ||
+            e.max_side == 0 || (e.lookahead == kLaFused && (e.width == SE_TUNE_U32 || e.pivot != 64))) {
+            set_error("tuning table: width must be SE_TUNE_F16/U16/U32, pivot 32 or 64, lookahead 0, 1 or 2 "
+                      "(2: 16-bit widths with pivot 64), max_side > 0");
             return SE_ERR_INVALID;
         }
     }

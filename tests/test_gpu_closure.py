@@ -79,7 +79,7 @@ def test_width_selection_is_exact(gpu_lib, oracle_mod, hi, width):
             _timed(d, _lib.SE_FLAG_FORCE_U16)
 
 
-@pytest.mark.parametrize("schedule", ["lookahead", "serial"])
+@pytest.mark.parametrize("schedule", ["fused", "lookahead", "serial"])
 @pytest.mark.parametrize("width", ["f16", "u16", "u32"])
 @pytest.mark.parametrize("pivot", [32, 64])
 @pytest.mark.parametrize("n", [1, 63, 64, 65, 200, 511, 640, 1500])
@@ -91,7 +91,8 @@ def test_every_width_and_pivot_depth(gpu_lib, oracle_mod, schedule, width, pivot
     d = rng.integers(0, hi, (n, n)).astype(np.uint32)
     flags = {"f16": 0, "u16": _lib.SE_FLAG_FORCE_U16 | _lib.SE_FLAG_NO_F16, "u32": _lib.SE_FLAG_FORCE_U32}[width]
     flags |= _lib.SE_FLAG_PIVOT32 if pivot == 32 else _lib.SE_FLAG_PIVOT64
-    flags |= _lib.SE_FLAG_NO_LOOKAHEAD if schedule == "serial" else _lib.SE_FLAG_LOOKAHEAD
+    flags |= {"serial": _lib.SE_FLAG_NO_LOOKAHEAD, "lookahead": _lib.SE_FLAG_LOOKAHEAD,
+              "fused": _lib.SE_FLAG_LOOKAHEAD_FUSED}[schedule]
     got, t = _timed(d, flags)
     assert (t["pivot"], t["width_bits"]) == (pivot, 32 if width == "u32" else 16)
     assert t["f16_ordered"] == (1 if (width == "f16" and n > 1) else 0)

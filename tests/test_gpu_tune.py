@@ -24,7 +24,7 @@ def test_table_choice_is_used_and_exact(gpu_lib, oracle_mod, width, n):
     d = rng.integers(0, 3000, (n, n)).astype(np.uint32)
     want = oracle_mod.closure_tiled(d, 16)
     try:
-        for pivot, la in autotune.CANDIDATES:
+        for pivot, la in autotune.candidates(width):
             print(f"tuned closure {width} n={n} pivot={pivot} lookahead={la}", flush=True)
             autotune.install([{"width": width, "max_side": autotune.ANY_SIDE, "pivot": pivot, "lookahead": la}])
             got, t = strongest_paths_timed(d, _flags(width))
@@ -38,8 +38,8 @@ def test_table_choice_is_used_and_exact(gpu_lib, oracle_mod, width, n):
 def test_table_buckets_select_by_padded_side(gpu_lib, oracle_mod):
     from scalingelections_amd import autotune, strongest_paths_timed
 
-    table = [{"width": "f16", "max_side": 256, "pivot": 32, "lookahead": True},
-             {"width": "f16", "max_side": 512, "pivot": 64, "lookahead": False}]
+    table = [{"width": "f16", "max_side": 256, "pivot": 32, "lookahead": 1},
+             {"width": "f16", "max_side": 512, "pivot": 64, "lookahead": 2}]
     try:
         autotune.install(table)
         for n, pivot in ((100, 32), (256, 32), (257, 64), (512, 64)):

@@ -312,10 +312,11 @@ def test_tuning_table_api():
     base = autotune.current_table()
     for w in ("f16", "u16", "u32"):
         assert any(e["width"] == w and e["max_side"] == autotune.ANY_SIDE for e in base), w
-        assert all(e["pivot"] in (32, 64) for e in base)
+        assert all(e["pivot"] in (32, 64) and e["lookahead"] in (0, 1, 2) for e in base)
+    assert autotune.candidates("u32") == ((32, 0), (32, 1), (64, 0), (64, 1))
     assert autotune.batch_rounds_min() == 129
     try:
-        mine = [{"width": "u32", "max_side": 512, "pivot": 64, "lookahead": True}]
+        mine = [{"width": "u32", "max_side": 512, "pivot": 64, "lookahead": 1}]
         autotune.install(mine, 1025)
         assert autotune.current_table() == mine and autotune.batch_rounds_min() == 1025
         L = _lib.lib()
@@ -325,6 +326,9 @@ def test_tuning_table_api():
         assert L.se_tune_closure(bad, 1) == _lib.SE_ERR_INVALID
         bad[0] = _lib.se_tune_entry(0, 0, 32, 0)
         assert L.se_tune_closure(bad, 1) == _lib.SE_ERR_INVALID
+        for w, piv, la in ((2, 64, 2), (0, 32, 2), (1, 64, 3)):  # one launch per round: 16-bit, pivot 64
+            bad[0] = _lib.se_tune_entry(w, 512, piv, la)
+            assert L.se_tune_closure(bad, 1) == _lib.SE_ERR_INVALID, (w, piv, la)
         assert L.se_tune_closure(None, 65) == _lib.SE_ERR_INVALID
         for th in (1, 128, 1026):
             assert L.se_tune_batch(th) == _lib.SE_ERR_INVALID and "129..1025" in _lib.last_error()
@@ -332,10 +336,10 @@ def test_tuning_table_api():
     finally:
         autotune.reset()
     assert autotune.current_table() == base and autotune.batch_rounds_min() == 129
-    got = autotune.buckets({1024: (64, True), 256: (32, True), 512: (32, True), 2048: (32, False)}, "u16")
-    assert got == [{"width": "u16", "max_side": 512, "pivot": 32, "lookahead": True},
-                   {"width": "u16", "max_side": 1024, "pivot": 64, "lookahead": True},
-                   {"width": "u16", "max_side": autotune.ANY_SIDE, "pivot": 32, "lookahead": False}]
+    got = autotune.buckets({1024: (64, 1), 256: (32, 1), 512: (32, 1), 2048: (32, 0)}, "u16")
+    assert got == [{"width": "u16", "max_side": 512, "pivot": 32, "lookahead": 1},
+                   {"width": "u16", "max_side": 1024, "pivot": 64, "lookahead": 1},
+                   {"width": "u16", "max_side": autotune.ANY_SIDE, "pivot": 32, "lookahead": 0}]
     assert autotune.padded_side(1) == 128 and autotune.padded_side(256) == 256 and autotune.padded_side(257) == 384
 
 
@@ -343,8 +347,8 @@ def test_tuning_table_save_load_roundtrip(tmp_path, monkeypatch):
     from scalingelections_amd import api, autotune
 
     monkeypatch.setattr(api, "device_info", lambda device=0: {"name": "test"})
-    entries = [{"width": "f16", "max_side": 1024, "pivot": 32, "lookahead": False},
-               {"width": "f16", "max_side": autotune.ANY_SIDE, "pivot": 64, "lookahead": True}]
+    entries = [{"width": "f16", "max_side": 1024, "pivot": 32, "lookahead": 0},
+               {"width": "f16", "max_side": autotune.ANY_SIDE, "pivot": 64, "lookahead": 1}]
     path = str(tmp_path / "tune.json")
     try:
         autotune.save(path, entries, 257, {"f16/256": {"T32": 0.1}})
