@@ -97,7 +97,7 @@ struct BatchLds {
     uint32_t rowbuf[2][T], colbuf[2][T];
     uint32_t wins[kMaxBatchC];
     union alignas(16) {
-        uint32_t S[kMacro * (T + 1)];
+        uint32_t S[kMacro * (T + 1) > 2 * 64 * 65 ? kMacro * (T + 1) : 2 * 64 * 65];
         E indep[sizeof(E) == 2 ? indep_u16_lds_elems<T>() : indep_u32_lds_elems<T>()];
     } u;
 };
@@ -108,6 +108,94 @@ __device__ __forceinline__ void indep_tile(E* W, size_t ldw, uint32_t kb, uint32
         indep_tile_u16<T, F16>(W, ldw, kb, I0, J0, lds);
     else
         indep_tile_u32<T>(W, ldw, kb, I0, J0, lds);
+}
+
+// Per-election square passes over 64 x 64 tile pairs (I, J), I <= J: the tile and its
+// transposed partner are staged in LDS (u32, pitch 65: conflict-free column reads), so both
+// orientations are read and written by whole rows (coalesced, no division per cell).  Wave
+// w of the workgroup handles rows w, w + 4, ... of a tile, lane c column c.
+constexpr uint32_t kPT = 64, kPP = kPT + 1;
+
+// p0 (scaling_elections.py:103-112) of one election (d: C x C) into its padded work matrix
+// (Cp x Cp): v = d[i][j] + bias when d[i][j] > d[j][i], else 0 (diagonal and padding 0).
+// A, B: 2 x 64 x 65 u32 of LDS.
+template <typename E>
+__device__ void batch_p0(const uint32_t* __restrict__ d, uint32_t C, E* __restrict__ W, uint32_t Cp,
+                         uint32_t bias, uint32_t* A, uint32_t* B) {
+    const uint32_t t = threadIdx.x, c = t & (kPT - 1), r0 = t / kPT;
+    const uint32_t nt = (Cp + kPT - 1) / kPT;
+    for (uint32_t I = 0; I < nt; ++I)
+        for (uint32_t J = I; J < nt; ++J) {
+            __syncthreads();
+            for (uint32_t r = r0; r < kPT; r += kThreads / kPT) {
+                const uint32_t i = I * kPT + r, j = J * kPT + c, i2 = J * kPT + r, j2 = I * kPT + c;
+                A[r * kPP + c] = (i < C && j < C) ? d[(size_t)i * C + j] : 0u;
+                B[r * kPP + c] = (i2 < C && j2 < C) ? d[(size_t)i2 * C + j2] : 0u;
+            }
+            __syncthreads();
+            for (uint32_t r = r0; r < kPT; r += kThreads / kPT) {
+                const uint32_t i = I * kPT + r, j = J * kPT + c;
+                if (i < Cp && j < Cp) {
+                    const uint32_t x = A[r * kPP + c], y = B[c * kPP + r];
+                    W[(size_t)i * Cp + j] = (E)((i != j && x > y) ? x + bias : 0u);
+                }
+                const uint32_t i2 = J * kPT + r, j2 = I * kPT + c;
+                if (I != J && i2 < Cp && j2 < Cp) {
+                    const uint32_t x = B[r * kPP + c], y = A[c * kPP + r];
+                    W[(size_t)i2 * Cp + j2] = (E)(x > y ? x + bias : 0u);
+                }
+            }
+        }
+    __syncthreads();
+}
+
+// Output p (C x C, zero diagonal, bias removed), wins[i] = #{j != i : W[i][j] > W[j][i]}
+// (scaling_elections.py:287-290; the offset keeps the order) and the stable ranking by
+// descending wins, ties in ascending index (:292): rank(i) = #{j : w_j > w_i or (w_j = w_i
+// and j < i)}.  wins: C words of LDS; A, B as in batch_p0.
+template <typename E>
+__device__ void batch_tail(const E* __restrict__ W, uint32_t Cp, uint32_t C, uint32_t bias, uint32_t* __restrict__ p,
+                           uint32_t* __restrict__ rank_out, uint32_t* wins, uint32_t* A, uint32_t* B) {
+    const uint32_t t = threadIdx.x, c = t & (kPT - 1), r0 = t / kPT;
+    for (uint32_t i = t; i < C; i += kThreads) wins[i] = 0;
+    const uint32_t nt = (C + kPT - 1) / kPT;
+    for (uint32_t I = 0; I < nt; ++I)
+        for (uint32_t J = I; J < nt; ++J) {
+            __syncthreads();
+            for (uint32_t r = r0; r < kPT; r += kThreads / kPT) {
+                const uint32_t i = I * kPT + r, j = J * kPT + c, i2 = J * kPT + r, j2 = I * kPT + c;
+                A[r * kPP + c] = (i < C && j < C) ? (uint32_t)W[(size_t)i * Cp + j] : 0u;
+                B[r * kPP + c] = (i2 < C && j2 < C) ? (uint32_t)W[(size_t)i2 * Cp + j2] : 0u;
+            }
+            __syncthreads();
+            uint32_t beats = 0;  // column j = J * 64 + c over this tile's rows (I != J)
+            for (uint32_t r = r0; r < kPT; r += kThreads / kPT) {
+                const uint32_t i = I * kPT + r, j = J * kPT + c;
+                const bool in = i < C && j < C;
+                const uint32_t x = A[r * kPP + c], y = B[c * kPP + r];
+                if (in) p[(size_t)i * C + j] = (i == j || x == 0) ? 0u : x - bias;
+                // row i is this wave's alone: count its wins with one ballot
+                const unsigned long long won = __ballot(in && i != j && x > y);
+                if (c == 0 && i < C) wins[i] += (uint32_t)__popcll(won);
+                if (I != J) {
+                    beats += (in && y > x) ? 1u : 0u;
+                    const uint32_t i2 = J * kPT + r, j2 = I * kPT + c;
+                    const uint32_t x2 = B[r * kPP + c];
+                    if (i2 < C && j2 < C) p[(size_t)i2 * C + j2] = x2 == 0 ? 0u : x2 - bias;
+                }
+            }
+            if (beats) atomicAdd(&wins[J * kPT + c], beats);
+        }
+    __syncthreads();
+    for (uint32_t i = t; i < C; i += kThreads) {
+        const uint32_t wi = wins[i];
+        uint32_t rank = 0;
+        for (uint32_t j = 0; j < C; ++j) {
+            const uint32_t wj = wins[j];
+            rank += (wj > wi || (wj == wi && j < i)) ? 1u : 0u;
+        }
+        rank_out[rank] = i;
+    }
 }
 
 // Offset of the f16-ordered batch closure: nonzero p0 values v <= m <= 30720 are stored as
@@ -138,17 +226,7 @@ __global__ __launch_bounds__(256) void batch_closure_kernel(const uint32_t* __re
     if (phase == 1 && alone) return;
     if (phase == 1) goto tail;
 
-    // p0 (scaling_elections.py:103-112) into the padded work matrix
-    for (uint32_t q = t; q < Cp * Cp; q += kThreads) {
-        const uint32_t i = q / Cp, j = q % Cp;
-        uint32_t v = 0;
-        if (i < C && j < C && i != j) {
-            const uint32_t x = d[(size_t)i * C + j];
-            if (x > d[(size_t)j * C + i]) v = x + bias;
-        }
-        W[q] = (E)v;
-    }
-    __syncthreads();
+    batch_p0<E>(d, C, W, Cp, bias, L.u.S, L.u.S + kPT * kPP);
     if (!alone) return;
 
     for (uint32_t r = 0, rounds = (C + T - 1) / T; r < rounds; ++r) {
@@ -176,28 +254,9 @@ __global__ __launch_bounds__(256) void batch_closure_kernel(const uint32_t* __re
     }
 
 tail:
-    // output with zero diagonal, then wins (:287-290) and the stable ranking (:292)
-    for (uint32_t q = t; q < C * C; q += kThreads) {
-        const uint32_t i = q / C, j = q % C;
-        const uint32_t w = (uint32_t)W[(size_t)i * Cp + j];
-        p[q] = (i == j || w == 0) ? 0u : w - bias;
-    }
-    for (uint32_t i = t; i < C; i += kThreads) {
-        uint32_t w = 0;
-        for (uint32_t j = 0; j < C; ++j)
-            if (j != i && W[(size_t)i * Cp + j] > W[(size_t)j * Cp + i]) ++w;
-        L.wins[i] = w;
-    }
-    __syncthreads();
-    for (uint32_t i = t; i < C; i += kThreads) {
-        const uint32_t wi = L.wins[i];
-        uint32_t rank = 0;
-        for (uint32_t j = 0; j < C; ++j) {
-            const uint32_t wj = L.wins[j];
-            rank += (wj > wi || (wj == wi && j < i)) ? 1u : 0u;
-        }
-        rank_all[em.r_off + rank] = i;
-    }
+    // output with zero diagonal, wins (:287-290) and the stable ranking (:292)
+    __syncthreads();  // the closure's last LDS use (the union) ends here
+    batch_tail<E>(W, Cp, C, bias, p, rank_all + em.r_off, L.wins, L.u.S, L.u.S + kPT * kPP);
 }
 
 // Round r of the round-closed elections, 16-bit: one workgroup per election for the
