@@ -313,6 +313,72 @@ __global__ __launch_bounds__(256) void panel_indep_u16_kernel(uint16_t* __restri
 }
 
 // ---------------------------------------------------------------------------------------
+// Small elections (n <= 128): the whole pipeline in one workgroup and one launch.  p0
+// (scaling_elections.py:103-112) straight into registers, 32-bit (so no width decision and
+// no host round trip), then Floyd-Warshall over the n pivots (:115-123), then the output
+// with its zero diagonal.  Lane (ty, tx) holds rows ty + 16 r and columns tx + 16 c
+// (r, c < 8).  Pivot k's row and column are published through LDS, double-buffered, so one
+// barrier per pivot: neither changes during step k (min(p[i][k], p[k][k]) <= p[i][k]).
+// The diagonal may pick up values meanwhile; like the reference's skipped diagonal it
+// never feeds an off-diagonal cell, and the output zeroes it.
+constexpr uint32_t kSmallN = 128;
+
+template <typename D, typename O>
+__global__ __launch_bounds__(256) void small_closure_kernel(const D* __restrict__ d, uint32_t n, size_t ld,
+                                                            O* __restrict__ out, size_t ldo) {
+    __shared__ uint32_t rowbuf[2][kSmallN], colbuf[2][kSmallN];
+    const uint32_t t = threadIdx.x, ty = t >> 4, tx = t & 15;
+    uint32_t cell[8][8];
+#pragma unroll
+    for (uint32_t r = 0; r < 8; ++r)
+#pragma unroll
+        for (uint32_t c = 0; c < 8; ++c) {
+            const uint32_t i = ty + 16 * r, j = tx + 16 * c;
+            uint32_t v = 0;
+            if (i < n && j < n && i != j) {
+                const uint32_t x = d[(size_t)i * ld + j];
+                if (x > (uint32_t)d[(size_t)j * ld + i]) v = x;
+            }
+            cell[r][c] = v;
+        }
+    // pivot k = 16 q + kk: its row lives at r = q in the lanes with ty == kk, its column at
+    // c = q in the lanes with tx == kk (register indices stay compile-time constants)
+#pragma unroll
+    for (uint32_t q = 0; q < 8; ++q) {
+        for (uint32_t kk = 0; kk < 16; ++kk) {
+            const uint32_t k = 16 * q + kk;
+            if (k >= n) break;
+            const uint32_t buf = k & 1;
+            if (ty == kk) {
+#pragma unroll
+                for (uint32_t c = 0; c < 8; ++c) rowbuf[buf][tx + 16 * c] = cell[q][c];
+            }
+            if (tx == kk) {
+#pragma unroll
+                for (uint32_t r = 0; r < 8; ++r) colbuf[buf][ty + 16 * r] = cell[r][q];
+            }
+            __syncthreads();
+            uint32_t a[8], b[8];
+#pragma unroll
+            for (uint32_t r = 0; r < 8; ++r) a[r] = colbuf[buf][ty + 16 * r];
+#pragma unroll
+            for (uint32_t c = 0; c < 8; ++c) b[c] = rowbuf[buf][tx + 16 * c];
+#pragma unroll
+            for (uint32_t r = 0; r < 8; ++r)
+#pragma unroll
+                for (uint32_t c = 0; c < 8; ++c) cell[r][c] = max(cell[r][c], min(a[r], b[c]));
+        }
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < 8; ++r)
+#pragma unroll
+        for (uint32_t c = 0; c < 8; ++c) {
+            const uint32_t i = ty + 16 * r, j = tx + 16 * c;
+            if (i < n && j < n) out[(size_t)i * ldo + j] = (O)(i == j ? 0u : cell[r][c]);
+        }
+}
+
+// ---------------------------------------------------------------------------------------
 // Output: out[i][j] = W[i][j] widened to u32, 0 on the diagonal, padding dropped.
 
 template <typename E, typename O>
@@ -547,6 +613,51 @@ int closure_device_ex(DeviceContext* ctx, const void* d_any, bool d16, uint32_t 
     const size_t ldw = N;
     const bool want_timing = timing != nullptr;
     EventTimer tm((flags & SE_FLAG_TIME_KERNELS) != 0, s);
+
+    // up to 128 candidates: one launch (small_closure_kernel), unless a flag asks for a
+    // particular width, pivot depth or schedule of the blocked path
+    constexpr uint32_t kShapeFlags = SE_FLAG_FORCE_U32 | SE_FLAG_FORCE_U16 | SE_FLAG_PIVOT32 | SE_FLAG_PIVOT64 |
+                                     SE_FLAG_NO_F16 | SE_FLAG_NO_LOOKAHEAD | SE_FLAG_LOOKAHEAD |
+                                     SE_FLAG_LOOKAHEAD_FUSED;
+    if (n <= kSmallN && !(flags & kShapeFlags)) {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (want_timing && (!hip_ok(hipEventCreate(&e0), "Failed to create events") ||
+                            !hip_ok(hipEventCreate(&e1), "Failed to create events")))
+            return SE_ERR_DEVICE;
+        if (want_timing) (void)hipEventRecord(e0, s);
+        // values of a 16-bit input fit the 16-bit output (p takes values of d only)
+        if (d16 && out16) {
+            small_closure_kernel<uint16_t, uint16_t>
+                <<<1, kThreads, 0, s>>>(d16p, n, ld, static_cast<uint16_t*>(out_any), ldo);
+            *out16 = true;
+        } else if (d16) {
+            small_closure_kernel<uint16_t, uint32_t>
+                <<<1, kThreads, 0, s>>>(d16p, n, ld, static_cast<uint32_t*>(out_any), ldo);
+        } else {
+            small_closure_kernel<uint32_t, uint32_t>
+                <<<1, kThreads, 0, s>>>(d32, n, ld, static_cast<uint32_t*>(out_any), ldo);
+        }
+        SE_CHECK_LAUNCH("closure launch failed");
+        if (want_timing) {
+            (void)hipEventRecord(e1, s);
+            if (!hip_ok(hipEventSynchronize(e1), "CUDA operations did not complete successfully"))
+                return SE_ERR_DEVICE;
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            se_timing t{};
+            t.total_ms = ms;
+            t.indep_ms = ms;
+            t.rounds = n;  // one pivot per step, no blocking
+            t.pivot = 1;
+            t.width_bits = 32;
+            t.padded_n = n;
+            t.indep_cells = (uint64_t)n * n * n;
+            *timing = t;
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        }
+        return SE_OK;
+    }
 
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (want_timing) {

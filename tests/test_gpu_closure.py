@@ -130,6 +130,28 @@ def test_f16_ordered_mode_boundaries(gpu_lib, oracle_mod, lo, hi, f16):
         assert t3["pivot"] == 32 and np.array_equal(got3, want)
 
 
+@pytest.mark.parametrize("hi", [300, 70000, 2**32 - 1])
+def test_small_elections_one_launch(gpu_lib, oracle_mod, hi):
+    """Up to 128 candidates the closure is one workgroup and one launch (32-bit in
+    registers, no width decision); values below and above 65535 (16- and 32-bit transit of
+    the host entry point), against the oracle; a shape flag selects the blocked path."""
+    from scalingelections_amd import _lib
+
+    for n in (1, 2, 3, 16, 17, 64, 100, 127, 128):
+        rng = np.random.default_rng(n * 31 + hi % 97)
+        d = rng.integers(0, hi, (n, n), dtype=np.uint64).astype(np.uint32)
+        want = oracle_mod.closure_serial(d)
+        got, t = _timed(d, 0)
+        assert (t["pivot"], t["rounds"], t["width_bits"]) == (1, n, 32), (n, t)
+        assert np.array_equal(got, want), (n, hi)
+        got2, t2 = _timed(d, _lib.SE_FLAG_FORCE_U32)
+        assert t2["pivot"] in (32, 64) and np.array_equal(got2, want), (n, hi)
+    n = 129  # first size on the blocked path
+    d = np.random.default_rng(5).integers(0, hi, (n, n), dtype=np.uint64).astype(np.uint32)
+    got, t = _timed(d, 0)
+    assert t["pivot"] in (32, 64) and np.array_equal(got, oracle_mod.closure_serial(d))
+
+
 def test_degenerate_inputs(gpu_lib, oracle_mod):
     for d in (np.zeros((64, 64), np.uint32), np.full((70, 70), 7, np.uint32),
               np.triu(np.full((130, 130), 9, np.uint32), 1), np.eye(40, dtype=np.uint32) * 5):
