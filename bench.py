@@ -489,25 +489,25 @@ def main() -> int:
             "parallelism": "single GPU",
         },
         "roofline": {
-            "kernel": (f"panel_indep_u16_kernel + indep_u16_kernel (independent tiles with the next panel, "
-                       f"{mode} instruction mix)" if width == 16 else
-                       "indep_u32_kernel (independent tiles, u32 instruction mix)"),
+            "kernel": (f"panel_indep_u16_kernel (each round's independent tiles with the next pivot's panel, "
+                       f"one launch per round) + indep_u16_kernel (last round), {mode} instruction mix"
+                       if width == 16 else "indep_u32_kernel (independent tiles, u32 instruction mix)"),
             "bound": "valu",
             "achieved": round(achieved, 3),
             "peak": round(peak, 2),
             "unit": "TCells/s (cell updates; 2 integer ops each)",
             "frac": round(achieved / peak, 4) if peak else None,
             "traffic": recorded_traffic(mode, n),
-            "traffic_note": "HBM bytes per round (both launches) from rocprofv3 FETCH_SIZE/WRITE_SIZE scaled by a "
-                            "1 GiB copy, profiles/panel_indep_f16_closure32768_hbm_traffic.md; algorithmic 4.295e9 "
+            "traffic_note": "HBM bytes per round from rocprofv3 FETCH_SIZE/WRITE_SIZE scaled by a 1 GiB copy, "
+                            "profiles/panel_indep_f16_closure32768_hbm_traffic.md; algorithmic 4.295e9 "
                             "(read + write the 16-bit matrix once)",
             "peak_basis": "256 CUs x 64 lanes x 2.4 GHz VALU issue x cell updates per lane-instruction "
                           "(f16-ordered 4/3, packed u16 1, u32 2/3)",
             "cells_per_round": cells_per_round,
             "avg_round_ms": round(avg_round_ms, 4),
             "rounds_timed": rounds_run,
-            "note": "per round the independent tiles run as two launches (next pivot's strips, then the rest); "
-                    "avg_round_ms sums both, from HIP events on the library stream",
+            "note": "avg_round_ms: the round's launches (one in the look-ahead schedule used at this size, two "
+                    "where the next pivot's strips run on their own), from HIP events on the library stream",
         },
         "phases_ms_per_step": {k: round(sum(t[k] for t in timings) / max(1, len(timings)), 2)
                                for k in ("total_ms", "init_ms", "panel_ms", "indep_ms", "output_ms")},
