@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <numeric>
 #include <type_traits>
@@ -58,36 +59,43 @@ struct ElectionMeta {
 constexpr uint32_t kNoRounds = 0xFFFFFFFFu;
 
 // ------------------------------------------------------------------------- counting
-// item.x = election, item.y = first row I0, item.z = first column J0 (candidates),
-// item.w = mode | rows << 8 | columns << 16; mode 1: full 128 x 128 tile, mode 0:
-// voter-parallel sub-block (small elections and the partial tiles at the edge).
+// One work item: rows [I0, I0 + ri) x columns [J0, J0 + rj) of election e's d, over voters
+// [vb, ve).  shape = mode | ri << 8 | rj << 16; mode 1: full 128 x 128 tile, mode 0:
+// voter-parallel sub-block (small elections and the partial tiles at the edge).  Items
+// whose work far exceeds a workgroup's fair share are cut into voter slices, which add
+// their counts atomically into the zeroed d (put = kPutAtomic); the others store.
+struct CountItem {
+    uint32_t e, I0, J0, shape;
+    uint32_t vb, ve, put, pad;
+};
+
 template <uint32_t K>
-__device__ __forceinline__ void batch_count_item(const int32_t* pos, uint32_t* d, uint32_t C, uint32_t m,
-                                                 const uint4 it, uint32_t (*PI)[kCT], uint32_t (*PJ)[kCT]) {
-    if ((it.w & 0xFFu) == 1) {
-        count_tile<K>(pos, 0, m, C, C, d, C, it.y / kCT, it.z / kCT, PI, PJ, &PI[0][0], false);
+__device__ __forceinline__ void batch_count_item(const int32_t* pos, uint32_t* d, uint32_t C, const CountItem& it,
+                                                 uint32_t (*PI)[kCT], uint32_t (*PJ)[kCT]) {
+    if ((it.shape & 0xFFu) == 1) {
+        count_tile<K>(pos, it.vb, it.ve, C, C, d, C, it.I0 / kCT, it.J0 / kCT, PI, PJ, &PI[0][0], it.put);
         return;
     }
-    const uint32_t ri = (it.w >> 8) & 0xFFu, rj = (it.w >> 16) & 0xFFu;
-    count_block<K>(pos, 0, m, C, it.y, it.z, ri, rj, d, C, PI, PJ, &PI[0][0], false);
+    const uint32_t ri = (it.shape >> 8) & 0xFFu, rj = (it.shape >> 16) & 0xFFu;
+    count_block<K>(pos, it.vb, it.ve, C, it.I0, it.J0, ri, rj, d, C, PI, PJ, &PI[0][0], it.put);
 }
 
 __global__ __launch_bounds__(256) void batch_count_kernel(const int32_t* __restrict__ pos_all,
                                                           uint32_t* __restrict__ d_all,
                                                           const ElectionMeta* __restrict__ meta,
-                                                          const uint4* __restrict__ items) {
+                                                          const CountItem* __restrict__ items) {
     __shared__ __align__(16) uint32_t PI[kVW][kCT];
     __shared__ __align__(16) uint32_t PJ[kVW][kCT];
-    const uint4 it = items[blockIdx.x];
-    const ElectionMeta em = meta[it.x];
+    const CountItem it = items[blockIdx.x];
+    const ElectionMeta em = meta[it.e];
     const int32_t* pos = pos_all + em.pos_off;
     uint32_t* d = d_all + em.d_off;
     if (em.K == 4)
-        batch_count_item<4>(pos, d, em.C, em.m, it, PI, PJ);
+        batch_count_item<4>(pos, d, em.C, it, PI, PJ);
     else if (em.K == 2)
-        batch_count_item<2>(pos, d, em.C, em.m, it, PI, PJ);
+        batch_count_item<2>(pos, d, em.C, it, PI, PJ);
     else
-        batch_count_item<1>(pos, d, em.C, em.m, it, PI, PJ);
+        batch_count_item<1>(pos, d, em.C, it, PI, PJ);
 }
 
 // ------------------------------------------------------------------------- closure
@@ -384,7 +392,7 @@ int batch_device_impl(DeviceContext* ctx, uint32_t E, const uint32_t* Cs, const 
             ro += em.C;
         }
         // work items, most expensive first
-        std::vector<uint4> items;
+        std::vector<CountItem> items;
         std::vector<double> cost;
         for (uint32_t e = 0; e < E; ++e) {
             const uint32_t C = Cs[e];
@@ -397,7 +405,7 @@ int batch_device_impl(DeviceContext* ctx, uint32_t E, const uint32_t* Cs, const 
             const uint32_t K = meta[e].K;
             const double words = (double)((ms[e] + K - 1) / K);
             auto add = [&](uint32_t I0, uint32_t J0, uint32_t ri, uint32_t rj, bool full) {
-                items.push_back(make_uint4(e, I0, J0, (full ? 1u : 0u) | (ri << 8) | (rj << 16)));
+                items.push_back(CountItem{e, I0, J0, (full ? 1u : 0u) | (ri << 8) | (rj << 16), 0u, ms[e], kPutStore, 0u});
                 // per-lane instructions: staging (K loads + packing per word) and 128 per
                 // word shared by the G word groups of a voter-parallel item
                 const uint32_t nbi = (ri + 7) / 8, nbj = (rj + 7) / 8;
@@ -413,10 +421,41 @@ int batch_device_impl(DeviceContext* ctx, uint32_t E, const uint32_t* Cs, const 
                     add(I0, J0, ri, rj, ri == kCT && rj == kCT);
                 }
         }
+        // Items far above a workgroup's fair share (total / kSliceShare) would set the
+        // launch's tail, running at a quarter speed beside three other workgroups on their
+        // CU: cut them into voter slices that add into a zeroed d atomically.
+        constexpr double kSliceShare = 2048.0;
+        constexpr uint32_t kMaxSlices = 32;
+        double total_cost = 0.0;
+        for (double c : cost) total_cost += c;
+        const double fair = total_cost / kSliceShare;
+        bool any_sliced = false;
+        for (size_t i = 0, n0 = items.size(); i < n0; ++i) {
+            const uint32_t m = items[i].ve;
+            const uint32_t S = (uint32_t)std::min<double>({std::ceil(cost[i] / fair), (double)kMaxSlices, (double)m});
+            if (S < 2) continue;
+            any_sliced = true;
+            const uint32_t step = (m + S - 1) / S;
+            const double part = cost[i] / S;
+            CountItem base = items[i];
+            base.put = kPutAtomic;
+            for (uint32_t v0 = 0, k = 0; v0 < m; v0 += step, ++k) {
+                CountItem it = base;
+                it.vb = v0;
+                it.ve = std::min(m, v0 + step);
+                if (k == 0) {
+                    items[i] = it;
+                    cost[i] = part;
+                } else {
+                    items.push_back(it);
+                    cost.push_back(part);
+                }
+            }
+        }
         std::vector<uint32_t> idx(items.size());
         std::iota(idx.begin(), idx.end(), 0u);
         std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
-        std::vector<uint4> sorted_items(items.size());
+        std::vector<CountItem> sorted_items(items.size());
         for (size_t i = 0; i < idx.size(); ++i) sorted_items[i] = items[idx[i]];
         std::vector<uint32_t> order(E);
         std::iota(order.begin(), order.end(), 0u);
@@ -436,24 +475,26 @@ int batch_device_impl(DeviceContext* ctx, uint32_t E, const uint32_t* Cs, const 
                 }
         std::sort(large.begin(), large.end(), [&](uint32_t a, uint32_t b) { return Cs[a] > Cs[b]; });
         const size_t meta_bytes = round_up(E * sizeof(ElectionMeta), 256);
-        const size_t item_bytes = round_up(std::max<size_t>(1, sorted_items.size()) * sizeof(uint4), 256);
+        const size_t item_bytes = round_up(std::max<size_t>(1, sorted_items.size()) * sizeof(CountItem), 256);
         const size_t order_bytes = round_up((size_t)E * 4, 256);
         char* mbuf = static_cast<char*>(scratch(ctx, kSlotBatchMeta, meta_bytes + item_bytes + order_bytes));
         uint32_t* dbuf = static_cast<uint32_t*>(scratch(ctx, kSlotBatchD, d_total * 4 + 256));
         void* wbuf = scratch(ctx, kSlotWork, wo * (use16 ? 2 : 4) + 256);
         if (!mbuf || !dbuf || !wbuf) return SE_ERR_ALLOC;
         ElectionMeta* dmeta = reinterpret_cast<ElectionMeta*>(mbuf);
-        uint4* ditems = reinterpret_cast<uint4*>(mbuf + meta_bytes);
+        CountItem* ditems = reinterpret_cast<CountItem*>(mbuf + meta_bytes);
         uint32_t* dorder = reinterpret_cast<uint32_t*>(mbuf + meta_bytes + item_bytes);
         if (!hip_ok(hipMemcpyAsync(dmeta, meta.data(), E * sizeof(ElectionMeta), hipMemcpyHostToDevice, s),
                     "Failed to copy data from host to device") ||
             (!sorted_items.empty() &&
-             !hip_ok(hipMemcpyAsync(ditems, sorted_items.data(), sorted_items.size() * sizeof(uint4),
+             !hip_ok(hipMemcpyAsync(ditems, sorted_items.data(), sorted_items.size() * sizeof(CountItem),
                                     hipMemcpyHostToDevice, s),
                      "Failed to copy data from host to device")) ||
             !hip_ok(hipMemcpyAsync(dorder, order.data(), (size_t)E * 4, hipMemcpyHostToDevice, s),
                     "Failed to copy data from host to device"))
             return SE_ERR_COPY;
+        if (any_sliced && !hip_ok(hipMemsetAsync(dbuf, 0, d_total * 4, s), "Failed to clear device memory"))
+            return SE_ERR_DEVICE;
         if (!sorted_items.empty())
             batch_count_kernel<<<(uint32_t)sorted_items.size(), kCThreads, 0, s>>>(pos, dbuf, dmeta, ditems);
         SE_CHECK_LAUNCH("batch counter launch failed");

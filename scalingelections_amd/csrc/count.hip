@@ -45,14 +45,14 @@ __global__ __launch_bounds__(256) void count_kernel(const int32_t* __restrict__ 
     uint32_t* o = out + (size_t)slice * plane;
     uint32_t bits;
     if (n <= kCT) {
-        bits = count_block<K>(pos, vb, ve, ldp, 0, 0, n, n, o, ldo, PI, PJ, &PI[0][0], accumulate != 0);
+        bits = count_block<K>(pos, vb, ve, ldp, 0, 0, n, n, o, ldo, PI, PJ, &PI[0][0], accumulate ? kPutAdd : kPutStore);
     } else {
         uint32_t I = 0;
         while (t >= nb - I) {
             t -= nb - I;
             ++I;
         }
-        bits = count_tile<K>(pos, vb, ve, n, ldp, o, ldo, I, I + t, PI, PJ, &PI[0][0], accumulate != 0);
+        bits = count_tile<K>(pos, vb, ve, n, ldp, o, ldo, I, I + t, PI, PJ, &PI[0][0], accumulate ? kPutAdd : kPutStore);
     }
     if (flag && (bits & PackTraits<K>::range_mask)) atomicOr(flag, 1u);
 }

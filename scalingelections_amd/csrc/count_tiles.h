@@ -95,10 +95,27 @@ __device__ __forceinline__ void tally_row(int32_t* acc, uint32_t x, const uint32
     }
 }
 
-__device__ __forceinline__ void put1(uint32_t* p, uint32_t v, bool accumulate) { *p = accumulate ? *p + v : v; }
-__device__ __forceinline__ void put4(uint32_t* p, uint4 v, bool accumulate) {
+// How counts reach d: stored, added (the only writer of those cells in this launch: a
+// later voter pass), or added atomically (voter slices of one tile running concurrently
+// into a zeroed d).
+constexpr uint32_t kPutStore = 0, kPutAdd = 1, kPutAtomic = 2;
+
+__device__ __forceinline__ void put1(uint32_t* p, uint32_t v, uint32_t mode) {
+    if (mode == kPutAtomic)
+        atomicAdd(p, v);
+    else
+        *p = mode == kPutAdd ? *p + v : v;
+}
+__device__ __forceinline__ void put4(uint32_t* p, uint4 v, uint32_t mode) {
+    if (mode == kPutAtomic) {
+        atomicAdd(p, v.x);
+        atomicAdd(p + 1, v.y);
+        atomicAdd(p + 2, v.z);
+        atomicAdd(p + 3, v.w);
+        return;
+    }
     uint4* q = reinterpret_cast<uint4*>(p);
-    if (accumulate) {
+    if (mode == kPutAdd) {
         const uint4 o = *q;
         v = make_uint4(o.x + v.x, o.y + v.y, o.z + v.z, o.w + v.w);
     }
@@ -199,13 +216,13 @@ __device__ __forceinline__ void reduce_sums(uint32_t* sums, const StageLane& L, 
 // written as the complement of the top-right quadrant, d[j][i] = m - d[i][j] (a strict
 // order per voter), which saves a quarter of the tile's counting.
 // PI/PJ: LDS staging of kVW words x kCT candidates; red: 2 x kCT words (may alias PI).
-// With `accumulate` the counts are added to d instead of stored.  Returns the OR of all
+// `put` (kPutStore / kPutAdd / kPutAtomic) says how counts reach d.  Returns the OR of all
 // staged positions (range check of the caller).
 template <uint32_t K>
 __device__ __forceinline__ uint32_t count_tile(const int32_t* __restrict__ pos, uint32_t vb, uint32_t ve,
                                                uint32_t n, size_t ldp, uint32_t* __restrict__ d, size_t ldd,
                                                uint32_t I, uint32_t J, uint32_t (*PI)[kCT], uint32_t (*PJ)[kCT],
-                                               uint32_t* red, bool accumulate) {
+                                               uint32_t* red, uint32_t put) {
     const uint32_t I0 = I * kCT, J0 = J * kCT;
     const uint32_t tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     const uint32_t ri = min(kCT, n - I0), rj = min(kCT, n - J0);
@@ -270,9 +287,9 @@ __device__ __forceinline__ uint32_t count_tile(const int32_t* __restrict__ pos, 
             const uint32_t i = I0 + (a < 4 ? 4 * ty + a : 64 + 4 * ty + (a - 4));
             if (a < 4 || !diag)
                 put4(d + (size_t)i * ldd + J0 + 4 * tx, make_uint4(out[a][0], out[a][1], out[a][2], out[a][3]),
-                     accumulate);
+                     put);
             put4(d + (size_t)i * ldd + J0 + 64 + 4 * tx, make_uint4(out[a][4], out[a][5], out[a][6], out[a][7]),
-                 accumulate);
+                 put);
         }
         if (diag) {  // bottom-left quadrant: complement of the top-right one
 #pragma unroll
@@ -280,7 +297,7 @@ __device__ __forceinline__ uint32_t count_tile(const int32_t* __restrict__ pos, 
                 const uint32_t j = J0 + 64 + 4 * tx + (b - 4);
                 put4(d + (size_t)j * ldd + I0 + 4 * ty,
                      make_uint4(mreal - out[0][b], mreal - out[1][b], mreal - out[2][b], mreal - out[3][b]),
-                     accumulate);
+                     put);
             }
         } else {
 #pragma unroll
@@ -288,10 +305,10 @@ __device__ __forceinline__ uint32_t count_tile(const int32_t* __restrict__ pos, 
                 const uint32_t j = J0 + (b < 4 ? 4 * tx + b : 64 + 4 * tx + (b - 4));
                 put4(d + (size_t)j * ldd + I0 + 4 * ty,
                      make_uint4(mreal - out[0][b], mreal - out[1][b], mreal - out[2][b], mreal - out[3][b]),
-                     accumulate);
+                     put);
                 put4(d + (size_t)j * ldd + I0 + 64 + 4 * ty,
                      make_uint4(mreal - out[4][b], mreal - out[5][b], mreal - out[6][b], mreal - out[7][b]),
-                     accumulate);
+                     put);
             }
         }
     } else {
@@ -305,9 +322,9 @@ __device__ __forceinline__ uint32_t count_tile(const int32_t* __restrict__ pos, 
                 if (lj >= rj) continue;
                 // diagonal tile: the bottom-left quadrant (a >= 4, b < 4) was not counted;
                 // it is written from the top-right one (a < 4, b >= 4) by complement
-                if (!diag || a < 4 || b >= 4) put1(d + (size_t)(I0 + li) * ldd + J0 + lj, out[a][b], accumulate);
+                if (!diag || a < 4 || b >= 4) put1(d + (size_t)(I0 + li) * ldd + J0 + lj, out[a][b], put);
                 if (!diag || (a < 4 && b >= 4))
-                    put1(d + (size_t)(J0 + lj) * ldd + I0 + li, mreal - out[a][b], accumulate);
+                    put1(d + (size_t)(J0 + lj) * ldd + I0 + li, mreal - out[a][b], put);
             }
         }
     }
@@ -321,13 +338,13 @@ __device__ __forceinline__ uint32_t count_tile(const int32_t* __restrict__ pos, 
 // may alias PI).  On a diagonal sub-block (I0 == J0) only blocks bi <= bj are counted; a
 // strict order per voter gives the lower ones by complement, d[j][i] = m - d[i][j] (the
 // diagonal counts itself: 0).  Off-diagonal sub-blocks write d[i][j] and, by complement,
-// d[j][i].  With `accumulate` the counts are added to d.  Returns the OR of all staged
+// d[j][i].  `put` says how counts reach d (put1).  Returns the OR of all staged
 // positions.
 template <uint32_t K>
 __device__ __forceinline__ uint32_t count_block(const int32_t* __restrict__ pos, uint32_t vb, uint32_t ve,
                                                 size_t ldp, uint32_t I0, uint32_t J0, uint32_t ri, uint32_t rj,
                                                 uint32_t* __restrict__ d, size_t ldd, uint32_t (*PI)[kCT],
-                                                uint32_t (*PJ)[kCT], uint32_t* red, bool accumulate) {
+                                                uint32_t (*PJ)[kCT], uint32_t* red, uint32_t put) {
     const uint32_t t = threadIdx.x;
     const bool diag = I0 == J0;
     const uint32_t nbi = (ri + 7) / 8, nbj = (rj + 7) / 8;
@@ -403,8 +420,8 @@ __device__ __forceinline__ uint32_t count_block(const int32_t* __restrict__ pos,
             if (li >= ri || lj >= rj) continue;
             const uint32_t v = finish<K>((int32_t)sum, sums[li], sums[kCT + lj], M);
             const uint32_t i = I0 + li, j = J0 + lj;
-            put1(d + (size_t)i * ldd + j, v, accumulate);
-            if (!(diag && qi == qj)) put1(d + (size_t)j * ldd + i, mreal - v, accumulate);  // diagonal block: both triangles counted
+            put1(d + (size_t)i * ldd + j, v, put);
+            if (!(diag && qi == qj)) put1(d + (size_t)j * ldd + i, mreal - v, put);  // diagonal block: both triangles counted
         }
     }
     return bits;
