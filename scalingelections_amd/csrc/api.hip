@@ -21,8 +21,9 @@ int closure_device_impl(DeviceContext* ctx, const uint32_t* d, uint32_t n, size_
                         size_t ldo, uint32_t flags, hipStream_t s, se_timing* timing);
 int closure_device_ex(DeviceContext* ctx, const void* d_any, bool d16, uint32_t n, size_t ld, void* out_any,
                       size_t ldo, bool* out16, uint32_t flags, hipStream_t s, se_timing* timing);
-int count_device_impl(DeviceContext* ctx, const int32_t* pos, uint32_t m, uint32_t n, uint32_t* d,
+int count_device_impl(DeviceContext* ctx, const void* pos, bool pos16, uint32_t m, uint32_t n, uint32_t* d,
                       size_t ldd, hipStream_t s, double* kernel_ms, uint32_t* flag);
+int narrow_counts16(const uint32_t* d, uint32_t n, size_t ldd, uint16_t* out, hipStream_t s);
 int clear_flag(uint32_t* flag, hipStream_t s);
 int count_rankings_device_impl(DeviceContext* ctx, const int32_t* ids, const int64_t* offsets,
                                uint32_t m, uint32_t n, uint32_t* d, size_t ldd, hipStream_t s);
@@ -302,17 +303,32 @@ int se_count(const int32_t* positions, uint32_t m, uint32_t n, uint32_t* prefs, 
     if (!ctx) return SE_ERR_NO_DEVICE;
     std::lock_guard<std::mutex> g(ctx->lock);
     hipStream_t s = ctx->stream;
+    // PCIe transit in 16 bits where the values allow: positions (a permutation of 0..n-1
+    // fits when n <= 65536; anything else fails the narrowing and goes up as int32, where
+    // the counter's range check reports it) and counts (at most m)
     const size_t pbytes = (size_t)m * n * sizeof(int32_t);
     const size_t dbytes = (size_t)n * n * sizeof(uint32_t);
-    int32_t* pos = static_cast<int32_t*>(scratch(ctx, kSlotAux, pbytes));
+    void* pos = scratch(ctx, kSlotAux, pbytes);
     if (!pos) return SE_ERR_ALLOC;
     uint32_t* d = static_cast<uint32_t*>(scratch(ctx, kSlotPrefs, dbytes));
     if (!d) return SE_ERR_ALLOC;
     uint32_t* flag = static_cast<uint32_t*>(scratch(ctx, kSlotCountFlag, 256));
     if (!flag) return SE_ERR_ALLOC;
-    int rc = m > 0 ? upload_rows(ctx, s, pos, positions, n * sizeof(int32_t), m, n * sizeof(int32_t)) : SE_OK;
+    const bool d16 = m <= 0xFFFFu;
+    uint16_t* dn = d16 ? static_cast<uint16_t*>(scratch(ctx, kSlotPaths, (size_t)n * n * sizeof(uint16_t))) : nullptr;
+    if (d16 && !dn) return SE_ERR_ALLOC;
+    bool pos16 = false;
+    int rc = SE_OK;
+    if (m > 0) {
+        rc = upload_rows_narrow16(ctx, s, static_cast<uint16_t*>(pos), reinterpret_cast<const uint32_t*>(positions),
+                                  n * sizeof(int32_t), m, n);
+        pos16 = rc == SE_OK;
+        if (rc == SE_NARROW_FAILED)
+            rc = upload_rows(ctx, s, pos, positions, n * sizeof(int32_t), m, n * sizeof(int32_t));
+    }
     if (rc == SE_OK) rc = clear_flag(flag, s);
-    if (rc == SE_OK) rc = count_device_impl(ctx, pos, m, n, d, n, s, nullptr, flag);
+    if (rc == SE_OK) rc = count_device_impl(ctx, pos, pos16, m, n, d, n, s, nullptr, flag);
+    if (rc == SE_OK && d16) rc = narrow_counts16(d, n, n, dn, s);
     if (rc != SE_OK) {
         (void)hipStreamSynchronize(s);
         return rc;
@@ -321,7 +337,8 @@ int se_count(const int32_t* positions, uint32_t m, uint32_t n, uint32_t* prefs, 
     if (!hip_ok(hipMemcpyAsync(&bad, flag, sizeof(bad), hipMemcpyDeviceToHost, s),
                 "Failed to copy data from device to host"))
         return SE_ERR_COPY;
-    rc = download_rows(ctx, s, prefs, prefs_stride * sizeof(uint32_t), d, n, n * sizeof(uint32_t));
+    rc = d16 ? download_rows_widen16(ctx, s, prefs, prefs_stride * sizeof(uint32_t), dn, n, n)
+             : download_rows(ctx, s, prefs, prefs_stride * sizeof(uint32_t), d, n, n * sizeof(uint32_t));
     if (rc != SE_OK) return rc;
     if (!hip_ok(hipStreamSynchronize(s), "CUDA transfers did not complete successfully")) return SE_ERR_COPY;
     if (bad) {
@@ -342,7 +359,8 @@ int se_count_device(const int32_t* positions_dev, uint32_t m, uint32_t n, uint32
     DeviceContext* ctx = context_for(stream);
     if (!ctx) return SE_ERR_NO_DEVICE;
     std::lock_guard<std::mutex> g(ctx->lock);
-    return count_device_impl(ctx, positions_dev, m, n, prefs_dev, prefs_stride, ctx->stream, kernel_ms, nullptr);
+    return count_device_impl(ctx, positions_dev, false, m, n, prefs_dev, prefs_stride, ctx->stream, kernel_ms,
+                             nullptr);
 }
 
 int se_count_rankings(const int32_t* ids, const int64_t* offsets, uint32_t m, uint32_t n, uint32_t* prefs,

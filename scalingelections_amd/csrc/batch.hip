@@ -35,7 +35,7 @@
 
 namespace se {
 
-int count_device_impl(DeviceContext* ctx, const int32_t* pos, uint32_t m, uint32_t n, uint32_t* d, size_t ldd,
+int count_device_impl(DeviceContext* ctx, const void* pos, bool pos16, uint32_t m, uint32_t n, uint32_t* d, size_t ldd,
                       hipStream_t s, double* kernel_ms, uint32_t* flag);
 int closure_device_impl(DeviceContext* ctx, const uint32_t* d, uint32_t n, size_t ld, uint32_t* out,
                         size_t ldo, uint32_t flags, hipStream_t s, se_timing* timing);
@@ -73,11 +73,11 @@ template <uint32_t K>
 __device__ __forceinline__ void batch_count_item(const int32_t* pos, uint32_t* d, uint32_t C, const CountItem& it,
                                                  uint32_t (*PI)[kCT], uint32_t (*PJ)[kCT]) {
     if ((it.shape & 0xFFu) == 1) {
-        count_tile<K>(pos, it.vb, it.ve, C, C, d, C, it.I0 / kCT, it.J0 / kCT, PI, PJ, &PI[0][0], it.put);
+        count_tile<K, int32_t>(pos, it.vb, it.ve, C, C, d, C, it.I0 / kCT, it.J0 / kCT, PI, PJ, &PI[0][0], it.put);
         return;
     }
     const uint32_t ri = (it.shape >> 8) & 0xFFu, rj = (it.shape >> 16) & 0xFFu;
-    count_block<K>(pos, it.vb, it.ve, C, it.I0, it.J0, ri, rj, d, C, PI, PJ, &PI[0][0], it.put);
+    count_block<K, int32_t>(pos, it.vb, it.ve, C, it.I0, it.J0, ri, rj, d, C, PI, PJ, &PI[0][0], it.put);
 }
 
 __global__ __launch_bounds__(256) void batch_count_kernel(const int32_t* __restrict__ pos_all,
@@ -347,7 +347,7 @@ int batch_device_impl(DeviceContext* ctx, uint32_t E, const uint32_t* Cs, const 
         for (uint32_t e = 0; e < E; ++e) {
             const uint32_t C = Cs[e], m = ms[e];
             if (C) {
-                int rc = count_device_impl(ctx, pos + po, m, C, dbuf, C, s, nullptr, nullptr);
+                int rc = count_device_impl(ctx, pos + po, false, m, C, dbuf, C, s, nullptr, nullptr);
                 if (rc == SE_OK) rc = closure_device_impl(ctx, dbuf, C, C, paths + doff, C, 0, s, nullptr);
                 if (rc == SE_OK) rc = wins_device_impl(ctx, paths + doff, C, C, wins, s);
                 if (rc != SE_OK) return rc;

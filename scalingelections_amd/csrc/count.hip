@@ -32,8 +32,8 @@ namespace {
 // slices each writes partial counts of its voters to its own n x n plane of `out` (the
 // complement d[j][i] = m_slice - d[i][j] holds per slice); reduce_slices_kernel sums the
 // planes.  Out-of-range positions for the packing width set *flag.
-template <uint32_t K>
-__global__ __launch_bounds__(256) void count_kernel(const int32_t* __restrict__ pos, uint32_t v0, uint32_t v1,
+template <uint32_t K, typename P>
+__global__ __launch_bounds__(256) void count_kernel(const P* __restrict__ pos, uint32_t v0, uint32_t v1,
                                                     uint32_t n, size_t ldp, uint32_t* __restrict__ out, size_t ldo,
                                                     uint32_t nb, uint32_t slices, uint32_t slice_len, size_t plane,
                                                     uint32_t accumulate, uint32_t* __restrict__ flag) {
@@ -45,14 +45,14 @@ __global__ __launch_bounds__(256) void count_kernel(const int32_t* __restrict__ 
     uint32_t* o = out + (size_t)slice * plane;
     uint32_t bits;
     if (n <= kCT) {
-        bits = count_block<K>(pos, vb, ve, ldp, 0, 0, n, n, o, ldo, PI, PJ, &PI[0][0], accumulate ? kPutAdd : kPutStore);
+        bits = count_block<K, P>(pos, vb, ve, ldp, 0, 0, n, n, o, ldo, PI, PJ, &PI[0][0], accumulate ? kPutAdd : kPutStore);
     } else {
         uint32_t I = 0;
         while (t >= nb - I) {
             t -= nb - I;
             ++I;
         }
-        bits = count_tile<K>(pos, vb, ve, n, ldp, o, ldo, I, I + t, PI, PJ, &PI[0][0], accumulate ? kPutAdd : kPutStore);
+        bits = count_tile<K, P>(pos, vb, ve, n, ldp, o, ldo, I, I + t, PI, PJ, &PI[0][0], accumulate ? kPutAdd : kPutStore);
     }
     if (flag && (bits & PackTraits<K>::range_mask)) atomicOr(flag, 1u);
 }
@@ -85,12 +85,24 @@ __global__ __launch_bounds__(256) void count_rankings_kernel(const int32_t* __re
     }
 }
 
-template <uint32_t K>
-void launch_count(uint32_t grid, hipStream_t s, const int32_t* pos, uint32_t v0, uint32_t v1, uint32_t n,
-                  size_t ldp, uint32_t* out, size_t ldo, uint32_t nb, uint32_t slices, uint32_t slice_len,
-                  size_t plane, uint32_t accumulate, uint32_t* flag) {
-    count_kernel<K><<<grid, kCThreads, 0, s>>>(pos, v0, v1, n, ldp, out, ldo, nb, slices, slice_len, plane,
-                                               accumulate, flag);
+template <uint32_t K, typename P>
+void launch_count(uint32_t grid, hipStream_t s, const P* pos, uint32_t v0, uint32_t v1, uint32_t n, size_t ldp,
+                  uint32_t* out, size_t ldo, uint32_t nb, uint32_t slices, uint32_t slice_len, size_t plane,
+                  uint32_t accumulate, uint32_t* flag) {
+    count_kernel<K, P><<<grid, kCThreads, 0, s>>>(pos, v0, v1, n, ldp, out, ldo, nb, slices, slice_len, plane,
+                                                  accumulate, flag);
+}
+
+template <typename P>
+void launch_count_k(uint32_t K, uint32_t grid, hipStream_t s, const P* pos, uint32_t v0, uint32_t v1, uint32_t n,
+                    uint32_t* out, size_t ldo, uint32_t nb, uint32_t slices, uint32_t slice_len, size_t plane,
+                    uint32_t accumulate, uint32_t* flag) {
+    if (K == 4)
+        launch_count<4, P>(grid, s, pos, v0, v1, n, n, out, ldo, nb, slices, slice_len, plane, accumulate, flag);
+    else if (K == 2)
+        launch_count<2, P>(grid, s, pos, v0, v1, n, n, out, ldo, nb, slices, slice_len, plane, accumulate, flag);
+    else
+        launch_count<1, P>(grid, s, pos, v0, v1, n, n, out, ldo, nb, slices, slice_len, plane, accumulate, flag);
 }
 
 }  // namespace
@@ -98,9 +110,11 @@ void launch_count(uint32_t grid, hipStream_t s, const int32_t* pos, uint32_t v0,
 // Counter plan: packing width K from n (bytes up to 128 candidates, 16-bit halves up to
 // 32768, int32 above), voter slices for parallelism (see below) capped at the largest
 // slice a K-packed accumulation keeps exact, and passes of at most `budget` bytes of
-// partial planes, each pass adding into d.
-int count_device_impl(DeviceContext* ctx, const int32_t* pos, uint32_t m, uint32_t n, uint32_t* d, size_t ldd,
-                      hipStream_t s, double* kernel_ms, uint32_t* flag) {
+// partial planes, each pass adding into d.  (Slices adding atomically into d instead
+// measured slower: at 512 candidates x 10000 voters about 100 slices pile some 64 atomic
+// adds onto every cell, 0.378 against 0.137 ms.)  Positions: int32, or uint16 with pos16.
+int count_device_impl(DeviceContext* ctx, const void* pos, bool pos16, uint32_t m, uint32_t n, uint32_t* d,
+                      size_t ldd, hipStream_t s, double* kernel_ms, uint32_t* flag) {
     if (n == 0) return SE_OK;
     const uint32_t nb = (n + kCT - 1) / kCT;
     const uint32_t tiles = n <= kCT ? 1 : nb * (nb + 1) / 2;
@@ -148,12 +162,12 @@ int count_device_impl(DeviceContext* ctx, const int32_t* pos, uint32_t m, uint32
         const size_t ldo = sl > 1 ? n : ldd, plane = sl > 1 ? (size_t)n * n : 0;
         const uint32_t acc = sl > 1 ? 0u : (p > 0 ? 1u : 0u);
         const uint32_t grid = tiles * sl;
-        if (K == 4)
-            launch_count<4>(grid, s, pos, v0, v1, n, n, out, ldo, nb, sl, (uint32_t)slice_len, plane, acc, flag);
-        else if (K == 2)
-            launch_count<2>(grid, s, pos, v0, v1, n, n, out, ldo, nb, sl, (uint32_t)slice_len, plane, acc, flag);
+        if (pos16)
+            launch_count_k<uint16_t>(K, grid, s, static_cast<const uint16_t*>(pos), v0, v1, n, out, ldo, nb, sl,
+                                     (uint32_t)slice_len, plane, acc, flag);
         else
-            launch_count<1>(grid, s, pos, v0, v1, n, n, out, ldo, nb, sl, (uint32_t)slice_len, plane, acc, flag);
+            launch_count_k<int32_t>(K, grid, s, static_cast<const int32_t*>(pos), v0, v1, n, out, ldo, nb, sl,
+                                    (uint32_t)slice_len, plane, acc, flag);
         SE_CHECK_LAUNCH("counter launch failed");
         if (sl > 1) {
             reduce_slices_kernel<<<n, 256, 0, s>>>(part, sl, plane, n, d, ldd, p > 0 ? 1u : 0u);
@@ -169,6 +183,21 @@ int count_device_impl(DeviceContext* ctx, const int32_t* pos, uint32_t m, uint32
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
     }
+    return SE_OK;
+}
+
+// d (n x n u32, row stride ldd) -> out (n x n u16, packed) for the host entry point's
+// 16-bit transit (every count is at most m <= 65535).
+__global__ __launch_bounds__(256) void narrow16_kernel(const uint32_t* __restrict__ d, uint32_t n, size_t ldd,
+                                                       uint16_t* __restrict__ out) {
+    const uint32_t i = blockIdx.x;
+    for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) out[(size_t)i * n + j] = (uint16_t)d[(size_t)i * ldd + j];
+}
+
+int narrow_counts16(const uint32_t* d, uint32_t n, size_t ldd, uint16_t* out, hipStream_t s) {
+    if (n == 0) return SE_OK;
+    narrow16_kernel<<<n, 256, 0, s>>>(d, n, ldd, out);
+    SE_CHECK_LAUNCH("counter launch failed");
     return SE_OK;
 }
 

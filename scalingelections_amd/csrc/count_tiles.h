@@ -166,8 +166,8 @@ __device__ __forceinline__ StageLane stage_lane(uint32_t ri, uint32_t rj, bool s
 // thread adds its positions to `s` (its share of the candidate's sum) and ORs them into
 // `bits` (range check).
 // SPLIT = false: one thread per candidate (G = 1), the loop the full tiles use.
-template <uint32_t K, bool SPLIT>
-__device__ __forceinline__ void stage_words(const int32_t* __restrict__ pos, size_t ldp, uint32_t vb, uint32_t ve,
+template <uint32_t K, bool SPLIT, typename Pos>
+__device__ __forceinline__ void stage_words(const Pos* __restrict__ pos, size_t ldp, uint32_t vb, uint32_t ve,
                                             uint32_t w0, uint32_t I0, uint32_t J0, const StageLane& L,
                                             uint32_t (*PI)[kCT], uint32_t (*PJ)[kCT], uint32_t& s,
                                             uint32_t& bits) {
@@ -175,7 +175,7 @@ __device__ __forceinline__ void stage_words(const int32_t* __restrict__ pos, siz
     const uint32_t base = L.side ? J0 : I0;
     uint32_t(*P)[kCT] = L.side ? PJ : PI;
     const uint32_t guard = L.side ? 0u : PackTraits<K>::guard;
-    const int32_t* col = pos + base + L.c;
+    const Pos* col = pos + base + L.c;
     const uint32_t first = SPLIT ? L.sub : 0u, step = SPLIT ? L.G : 1u;
 #pragma unroll 4
     for (uint32_t w = first; w < kVW; w += step) {
@@ -218,8 +218,9 @@ __device__ __forceinline__ void reduce_sums(uint32_t* sums, const StageLane& L, 
 // PI/PJ: LDS staging of kVW words x kCT candidates; red: 2 x kCT words (may alias PI).
 // `put` (kPutStore / kPutAdd / kPutAtomic) says how counts reach d.  Returns the OR of all
 // staged positions (range check of the caller).
-template <uint32_t K>
-__device__ __forceinline__ uint32_t count_tile(const int32_t* __restrict__ pos, uint32_t vb, uint32_t ve,
+// Positions P: int32 (any caller) or uint16 (the host entry point's 16-bit transit).
+template <uint32_t K, typename P>
+__device__ __forceinline__ uint32_t count_tile(const P* __restrict__ pos, uint32_t vb, uint32_t ve,
                                                uint32_t n, size_t ldp, uint32_t* __restrict__ d, size_t ldd,
                                                uint32_t I, uint32_t J, uint32_t (*PI)[kCT], uint32_t (*PJ)[kCT],
                                                uint32_t* red, uint32_t put) {
@@ -241,7 +242,7 @@ __device__ __forceinline__ uint32_t count_tile(const int32_t* __restrict__ pos, 
         constexpr bool DIAG = decltype(diag_tag)::value;
         for (uint32_t w0 = 0; w0 < words; w0 += kVW) {
             __syncthreads();
-            stage_words<K, false>(pos, ldp, vb, ve, w0, I0, J0, SL, PI, PJ, s, bits);
+            stage_words<K, false, P>(pos, ldp, vb, ve, w0, I0, J0, SL, PI, PJ, s, bits);
             __syncthreads();
             const uint32_t wend = min(kVW, words - w0);
             for (uint32_t w = 0; w < wend; ++w) {
@@ -340,8 +341,8 @@ __device__ __forceinline__ uint32_t count_tile(const int32_t* __restrict__ pos, 
 // diagonal counts itself: 0).  Off-diagonal sub-blocks write d[i][j] and, by complement,
 // d[j][i].  `put` says how counts reach d (put1).  Returns the OR of all staged
 // positions.
-template <uint32_t K>
-__device__ __forceinline__ uint32_t count_block(const int32_t* __restrict__ pos, uint32_t vb, uint32_t ve,
+template <uint32_t K, typename P>
+__device__ __forceinline__ uint32_t count_block(const P* __restrict__ pos, uint32_t vb, uint32_t ve,
                                                 size_t ldp, uint32_t I0, uint32_t J0, uint32_t ri, uint32_t rj,
                                                 uint32_t* __restrict__ d, size_t ldd, uint32_t (*PI)[kCT],
                                                 uint32_t (*PJ)[kCT], uint32_t* red, uint32_t put) {
@@ -374,7 +375,7 @@ __device__ __forceinline__ uint32_t count_block(const int32_t* __restrict__ pos,
     const StageLane SL = stage_lane(ri, rj, true);
     for (uint32_t w0 = 0; w0 < words; w0 += kVW) {
         __syncthreads();
-        stage_words<K, true>(pos, ldp, vb, ve, w0, I0, J0, SL, PI, PJ, s, bits);
+        stage_words<K, true, P>(pos, ldp, vb, ve, w0, I0, J0, SL, PI, PJ, s, bits);
         __syncthreads();
         if (g < G) {
             const uint32_t wend = min(kVW, words - w0);

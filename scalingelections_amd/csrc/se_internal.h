@@ -26,14 +26,14 @@ extern std::atomic<int> g_interrupted;
 enum Slot : int {
     kSlotWork = 0,     // padded closure matrix (u16 or u32)
     kSlotPrefs = 1,    // staged input matrix (host entry points)
-    kSlotPaths = 2,    // staged output matrix (host entry points)
+    kSlotPaths = 2,    // staged output matrix (host entry points; se_count's 16-bit counts)
     kSlotSmall = 3,    // scalars: max(p0), counters
     kSlotPivot = 4,    // pivot-tile hand-off between kernels
     kSlotAux = 5,      // positions / wins staging
     kSlotBatchD = 6,   // batched preference matrices
     kSlotBatchMeta = 7,  // batch metadata / per-launch wins
-    kSlotCountPart = 8,  // per-slice partial preference matrices
-    kSlotCountFlag = 9,  // counter range-check flag
+    kSlotCountPart = 8,     // per-slice partial preference matrices
+    kSlotCountFlag = 9,     // counter range-check flag
     kSlotBatchPivot = 10,   // per-election pivot hand-off of the round-closed batch elections
     kSlotBatchRounds = 11,  // their per-round work lists
     kSlotCount = 12

@@ -261,6 +261,21 @@ def test_counter_passes_beyond_16bit_slices(gpu_lib, monkeypatch, budget_planes)
     assert np.array_equal(d.astype(np.int64) + d.T, m * (1 - np.eye(n, dtype=np.int64)))
 
 
+@pytest.mark.parametrize("n,m", [(130, 140000), (100, 70000), (300, 66000), (300, 65535)])
+def test_counter_slices_beyond_16bit_range(gpu_lib, n, m):
+    """More voters than one packed slice holds (65534 at 16 bits): the voter slices' partial
+    planes are summed; also the 8-bit packing (n <= 128) with many voters, counts above
+    65535 (32-bit transit back to the host) and at exactly 65535 (16-bit transit)."""
+    from scalingelections_amd import count_positions
+
+    print(f"counter slices n={n} m={m}", flush=True)
+    rng = np.random.default_rng(11 + n)
+    pos = np.argsort(rng.random((m, n)), axis=1).astype(np.int32)
+    d = count_positions(pos)
+    assert np.array_equal(d, _count_np(pos))
+    assert np.array_equal(d.astype(np.int64) + d.T, m * (1 - np.eye(n, dtype=np.int64)))
+
+
 @pytest.mark.parametrize("n,bad", [(100, 128), (100, -1), (200, 40000), (200, -5)])
 def test_counter_rejects_positions_outside_the_packed_range(gpu_lib, n, bad):
     """Positions must be a permutation of 0..n-1; values the packed counter cannot hold
@@ -304,3 +319,31 @@ def test_log_gpus_reports_gfx950(gpu_lib, capsys):
     out = capsys.readouterr().out
     assert "Device 0" in out
     assert device_info(0)["arch"].startswith("gfx950")
+
+
+@pytest.mark.parametrize("argv,n", [
+    (["--num-candidates", "128", "--num-voters", "2000", "--seed", "1"], 128),  # the repo default
+    (["--num-candidates", "300", "--num-voters", "0", "--seed", "7"], 300),      # random matrix
+])
+def test_cli_end_to_end(gpu_lib, oracle_mod, capsys, argv, n):
+    """The reference's benchmark CLI (scaling_elections.py:361-565) on the GPU: the serial
+    baseline, the tiled CPU branch and both GPU entries run, every one is validated against
+    the baseline, and the reported winner and top 5 are the oracle's."""
+    from scalingelections_amd import cli
+
+    rc = cli.main(argv + ["--run-cpu", "--run-gpu", "--warmup", "1", "--repeat", "2"])
+    out = capsys.readouterr().out
+    assert rc == 0, out
+    assert out.count("✓ Results validated") == 3 and "✗" not in out, out
+    for line in ("→ Serial (C++)", "→ Tiled CPU (C++ threads)", "→ Tiled GPU (HIP)", "(avg of 2)"):
+        assert line in out, line
+    seed = int(argv[argv.index("--seed") + 1])
+    rng = np.random.default_rng(seed)
+    if "--num-voters" in argv and argv[argv.index("--num-voters") + 1] == "0":
+        d = rng.integers(0, n, (n, n)).astype(np.uint32)
+    else:
+        m = int(argv[argv.index("--num-voters") + 1])
+        d = oracle_mod.build_prefs([rng.permutation(n) for _ in range(m)])
+    r = oracle_mod.ranking(oracle_mod.closure_serial(d))
+    assert f"Winner: Candidate #{int(r[0])}" in out
+    assert f"Top 5:  #{int(r[0])}, #{int(r[1])}, #{int(r[2])}, #{int(r[3])}, #{int(r[4])}" in out
