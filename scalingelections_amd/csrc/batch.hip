@@ -1,22 +1,27 @@
-// batch.hip -- thousands of independent elections of uneven size in two launches.
+// batch.hip -- thousands of independent elections of uneven size in a few launches.
 //
 // Per election e (C_e candidates, m_e voters, positions m_e x C_e int32) the batch
 // produces exactly what the single-election path produces: the strongest-path matrix
 // (scaling_elections.py:91-125 semantics, zero diagonal) and the ranking by descending
 // wins with ties in ascending index (:274-296).
 //
-// Launch 1, batch_count_kernel: one workgroup per work item, items sorted by decreasing
+// Counting, batch_count_kernel: one workgroup per work item, items sorted by decreasing
 //   cost so the longest start first.  Elections with C <= 128 are one item each, counted
 //   voter-parallel: the C x C counter matrix is cut into 8 x 8 blocks (<= 256) and the
 //   256 lanes are split into G = 256 / blocks voter groups; partial counts are summed
 //   through LDS.  Larger elections are cut into 128 x 128 tile pairs (I <= J), each an
 //   item, counted like the single-election kernel (count_tile), d[j][i] = m - d[i][j].
-// Launch 2, batch_closure_kernel: one workgroup per election, largest first.  p0 into a
+//   Items far above a workgroup's fair share are cut into voter slices that add into a
+//   zeroed d atomically.
+// Closure, batch_closure_kernel: one workgroup per election, largest first.  p0 into a
 //   zero-padded work matrix (global memory, L2-resident at these sizes), then the blocked
 //   closure run by the workgroup alone (close_pivot, panel chunk products, independent
 //   macro tiles -- the same building blocks as the single-election kernels, with barriers
 //   in place of kernel boundaries), then output, win counts and the stable ranking, all
-//   on the device.  Elections with C <= 64 are closed entirely in registers/LDS.
+//   on the device.  Elections with C <= 64 are closed entirely in registers/LDS.  On the
+//   16-bit paths the elections above a threshold (launch-shape table, 129 by default) are
+//   instead closed together by one pair of launches per round (batch_panel_kernel, then
+//   batch_tile_kernel over all their macro tiles), followed by a tail phase.
 //
 // The per-launch strategy (SE_BATCH_PER_LAUNCH) runs every election through the plain
 // single-election entry points instead, for comparison.
