@@ -146,7 +146,8 @@ int se_rank_device(const uint32_t *paths_dev, uint32_t n, size_t row_stride, uin
  * scaling_elections.py:51-296); this entry point runs the same three steps for many
  * elections at once and returns, per election, exactly what those calls return.
  * candidates/voters: host arrays of `elections` entries (C_e <= 1024; m_e voters).
- * positions_dev: concatenation over e of m_e x C_e int32 position arrays (strict orders).
+ * positions_dev: concatenation over e of m_e x C_e position arrays (strict orders), int32,
+ * or uint16 with SE_BATCH_POS16.
  * paths_dev: concatenation of C_e x C_e uint32 strongest-path matrices (zero diagonal).
  * ranking_dev: concatenation of C_e candidate indices by descending wins, ties ascending.
  * Default strategy: two launches for the whole batch (counting work items, then one
@@ -157,6 +158,9 @@ int se_rank_device(const uint32_t *paths_dev, uint32_t n, size_t row_stride, uin
  * default share per-round launches (panels, then tiles) with the batch's other large
  * elections.  Results never change; kept for comparison. */
 #define SE_BATCH_ONE_GROUP 1024u
+/* positions_dev holds uint16 positions instead of int32 (every position of a batched
+ * election is below 1024): half the bytes the counter reads.  Results never change. */
+#define SE_BATCH_POS16 2048u
 
 typedef struct se_batch_timing {
     double total_ms;          /* counting + closure                                    */
@@ -168,8 +172,13 @@ typedef struct se_batch_timing {
     uint64_t pair_votes;      /* sum of m * C * (C - 1) / 2                            */
 } se_batch_timing;
 
+/* Device-side conversion of int32 positions (count values) to the uint16 form
+ * SE_BATCH_POS16 takes; synchronous.  SE_ERR_INVALID when a value lies outside 0..65535
+ * (dst then holds its low 16 bits).  No reference counterpart (an input format). */
+int se_positions16_device(const int32_t *src_dev, size_t count, uint16_t *dst_dev, void *stream);
+
 int se_batch_device(uint32_t elections, const uint32_t *candidates, const uint32_t *voters,
-                    const int32_t *positions_dev, uint32_t *paths_dev, uint32_t *ranking_dev,
+                    const void *positions_dev, uint32_t *paths_dev, uint32_t *ranking_dev,
                     uint32_t flags, void *stream, se_batch_timing *timing);
 
 /* ---- launch-shape tuning (no reference counterpart) ------------------------------------ */

@@ -44,6 +44,7 @@ EXPORTED = (
     "se_interrupt", "se_clear_interrupt", "se_install_sigint", "se_restore_sigint",
     "se_batch_device", "se_synth_mallows", "se_stream_release",
     "se_tune_closure", "se_tune_closure_get", "se_tune_batch", "se_tune_batch_get",
+    "se_positions16_device",
 )
 
 SE_TUNE_F16 = 0
@@ -53,6 +54,7 @@ SE_TUNE_MAX_ENTRIES = 64
 
 SE_BATCH_PER_LAUNCH = 64
 SE_BATCH_ONE_GROUP = 1024
+SE_BATCH_POS16 = 2048
 
 
 class se_batch_timing(ctypes.Structure):
@@ -158,6 +160,8 @@ def _declare(L):
     L.se_synth_mallows.restype = c.c_int
     L.se_stream_release.argtypes = [vp]
     L.se_stream_release.restype = c.c_int
+    L.se_positions16_device.argtypes = [vp, sz, vp, vp]
+    L.se_positions16_device.restype = c.c_int
     L.se_tune_closure.argtypes = [c.POINTER(se_tune_entry), u32]
     L.se_tune_closure.restype = c.c_int
     L.se_tune_closure_get.argtypes = [c.POINTER(se_tune_entry), u32, c.POINTER(u32)]

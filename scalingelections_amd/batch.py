@@ -85,11 +85,24 @@ def mixed_size_workload(elections: int = 4096, seed: int = 4, c_range=(8, 512), 
 
 
 class DeviceBatch:
-    """A workload uploaded to the device with output buffers, ready to run repeatedly."""
+    """A workload uploaded to the device with output buffers, ready to run repeatedly.
 
-    def __init__(self, w: Workload):
+    pos16 (default): the positions are kept on the device as uint16 (converted there by
+    se_positions16_device) and the batch runs with SE_BATCH_POS16, halving the bytes the
+    counter reads; False keeps the int32 positions.  Results are the same either way."""
+
+    def __init__(self, w: Workload, pos16: bool = True):
         self.w = w
-        self.pos = devmem.DeviceBuffer.from_array(w.positions) if w.positions.size else devmem.DeviceBuffer(4)
+        self.pos16 = bool(pos16) and w.positions.size > 0
+        if not w.positions.size:
+            self.pos = devmem.DeviceBuffer(4)
+        elif self.pos16:
+            raw = devmem.DeviceBuffer.from_array(w.positions)
+            self.pos = devmem.DeviceBuffer(w.positions.size * 2)
+            _lib.check(_lib.lib().se_positions16_device(raw.ptr, w.positions.size, self.pos.ptr, None))
+            raw.close()
+        else:
+            self.pos = devmem.DeviceBuffer.from_array(w.positions)
         self.paths = devmem.DeviceBuffer(int(w.paths_offsets()[-1]) * 4 + 4)
         self.ranking = devmem.DeviceBuffer(int(w.rank_offsets()[-1]) * 4 + 4)
         self.C = np.ascontiguousarray(w.C, dtype=np.uint32)
@@ -97,7 +110,8 @@ class DeviceBatch:
 
     def run(self, per_launch: bool = False, stream=None, one_group: bool = False) -> dict:
         t = _lib.se_batch_timing()
-        flags = (_lib.SE_BATCH_PER_LAUNCH if per_launch else 0) | (_lib.SE_BATCH_ONE_GROUP if one_group else 0)
+        flags = (_lib.SE_BATCH_PER_LAUNCH if per_launch else 0) | (_lib.SE_BATCH_ONE_GROUP if one_group else 0) \
+            | (_lib.SE_BATCH_POS16 if self.pos16 else 0)
         rc = _lib.lib().se_batch_device(self.C.shape[0], _lib.ptr(self.C), _lib.ptr(self.m), self.pos.ptr,
                                         self.paths.ptr, self.ranking.ptr, flags,
                                         ctypes.c_void_p(stream.handle) if stream else None, ctypes.byref(t))

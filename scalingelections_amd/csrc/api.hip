@@ -24,12 +24,13 @@ int closure_device_ex(DeviceContext* ctx, const void* d_any, bool d16, uint32_t 
 int count_device_impl(DeviceContext* ctx, const void* pos, bool pos16, uint32_t m, uint32_t n, uint32_t* d,
                       size_t ldd, hipStream_t s, double* kernel_ms, uint32_t* flag);
 int narrow_counts16(const uint32_t* d, uint32_t n, size_t ldd, uint16_t* out, hipStream_t s);
+int narrow_positions16(const int32_t* src, size_t count, uint16_t* dst, uint32_t* flag, hipStream_t s);
 int clear_flag(uint32_t* flag, hipStream_t s);
 int count_rankings_device_impl(DeviceContext* ctx, const int32_t* ids, const int64_t* offsets,
                                uint32_t m, uint32_t n, uint32_t* d, size_t ldd, hipStream_t s);
 int wins_device_impl(DeviceContext* ctx, const uint32_t* p, uint32_t n, size_t ld, uint32_t* wins,
                      hipStream_t s);
-int batch_device_impl(DeviceContext* ctx, uint32_t E, const uint32_t* Cs, const uint32_t* ms, const int32_t* pos,
+int batch_device_impl(DeviceContext* ctx, uint32_t E, const uint32_t* Cs, const uint32_t* ms, const void* pos,
                       uint32_t* paths, uint32_t* ranking, uint32_t flags, hipStream_t s, se_batch_timing* timing);
 int closure_cpu_impl(const uint32_t* d, uint32_t n, size_t ld, uint32_t* p, uint32_t tile, int threads);
 int closure_cpu_serial_impl(const uint32_t* d, uint32_t n, size_t ld, uint32_t* p);
@@ -467,7 +468,7 @@ int se_rank(const uint32_t* paths, uint32_t n, size_t row_stride, uint32_t* wins
 }
 
 int se_batch_device(uint32_t elections, const uint32_t* candidates, const uint32_t* voters,
-                    const int32_t* positions_dev, uint32_t* paths_dev, uint32_t* ranking_dev, uint32_t flags,
+                    const void* positions_dev, uint32_t* paths_dev, uint32_t* ranking_dev, uint32_t flags,
                     void* stream, se_batch_timing* timing) {
     clear_error();
     if (timing) *timing = se_batch_timing{};
@@ -487,6 +488,34 @@ int se_batch_device(uint32_t elections, const uint32_t* candidates, const uint32
     std::lock_guard<std::mutex> g(ctx->lock);
     return batch_device_impl(ctx, elections, candidates, voters, positions_dev, paths_dev, ranking_dev, flags,
                              ctx->stream, timing);
+}
+
+int se_positions16_device(const int32_t* src_dev, size_t count, uint16_t* dst_dev, void* stream) {
+    clear_error();
+    if (count == 0) return SE_OK;
+    if (!src_dev || !dst_dev) {
+        set_error("positions buffers must not be NULL");
+        return SE_ERR_INVALID;
+    }
+    DeviceContext* ctx = context_for(stream);
+    if (!ctx) return SE_ERR_NO_DEVICE;
+    std::lock_guard<std::mutex> g(ctx->lock);
+    hipStream_t s = ctx->stream;
+    uint32_t* flag = static_cast<uint32_t*>(scratch(ctx, kSlotCountFlag, 256));
+    if (!flag) return SE_ERR_ALLOC;
+    int rc = clear_flag(flag, s);
+    if (rc == SE_OK) rc = narrow_positions16(src_dev, count, dst_dev, flag, s);
+    if (rc != SE_OK) return rc;
+    uint32_t bad = 0;
+    if (!hip_ok(hipMemcpyAsync(&bad, flag, sizeof(bad), hipMemcpyDeviceToHost, s),
+                "Failed to copy data from device to host") ||
+        !hip_ok(hipStreamSynchronize(s), "CUDA operations did not complete successfully"))
+        return SE_ERR_COPY;
+    if (bad) {
+        set_error("positions out of range: a value lies outside 0..65535");
+        return SE_ERR_INVALID;
+    }
+    return SE_OK;
 }
 
 float se_reduce_f32(const float* data, size_t count) {

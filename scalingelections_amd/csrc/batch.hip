@@ -74,18 +74,20 @@ struct CountItem {
     uint32_t vb, ve, put, pad;
 };
 
-template <uint32_t K>
-__device__ __forceinline__ void batch_count_item(const int32_t* pos, uint32_t* d, uint32_t C, const CountItem& it,
+template <uint32_t K, typename P>
+__device__ __forceinline__ void batch_count_item(const P* pos, uint32_t* d, uint32_t C, const CountItem& it,
                                                  uint32_t (*PI)[kCT], uint32_t (*PJ)[kCT]) {
     if ((it.shape & 0xFFu) == 1) {
-        count_tile<K, int32_t>(pos, it.vb, it.ve, C, C, d, C, it.I0 / kCT, it.J0 / kCT, PI, PJ, &PI[0][0], it.put);
+        count_tile<K, P>(pos, it.vb, it.ve, C, C, d, C, it.I0 / kCT, it.J0 / kCT, PI, PJ, &PI[0][0], it.put);
         return;
     }
     const uint32_t ri = (it.shape >> 8) & 0xFFu, rj = (it.shape >> 16) & 0xFFu;
-    count_block<K, int32_t>(pos, it.vb, it.ve, C, it.I0, it.J0, ri, rj, d, C, PI, PJ, &PI[0][0], it.put);
+    count_block<K, P>(pos, it.vb, it.ve, C, it.I0, it.J0, ri, rj, d, C, PI, PJ, &PI[0][0], it.put);
 }
 
-__global__ __launch_bounds__(256) void batch_count_kernel(const int32_t* __restrict__ pos_all,
+// Positions P: int32, or uint16 (SE_BATCH_POS16: half the bytes the staging reads).
+template <typename P>
+__global__ __launch_bounds__(256) void batch_count_kernel(const P* __restrict__ pos_all,
                                                           uint32_t* __restrict__ d_all,
                                                           const ElectionMeta* __restrict__ meta,
                                                           const CountItem* __restrict__ items) {
@@ -93,14 +95,14 @@ __global__ __launch_bounds__(256) void batch_count_kernel(const int32_t* __restr
     __shared__ __align__(16) uint32_t PJ[kVW][kCT];
     const CountItem it = items[blockIdx.x];
     const ElectionMeta em = meta[it.e];
-    const int32_t* pos = pos_all + em.pos_off;
+    const P* pos = pos_all + em.pos_off;
     uint32_t* d = d_all + em.d_off;
     if (em.K == 4)
-        batch_count_item<4>(pos, d, em.C, it, PI, PJ);
+        batch_count_item<4, P>(pos, d, em.C, it, PI, PJ);
     else if (em.K == 2)
-        batch_count_item<2>(pos, d, em.C, it, PI, PJ);
+        batch_count_item<2, P>(pos, d, em.C, it, PI, PJ);
     else
-        batch_count_item<1>(pos, d, em.C, it, PI, PJ);
+        batch_count_item<1, P>(pos, d, em.C, it, PI, PJ);
 }
 
 // ------------------------------------------------------------------------- closure
@@ -319,10 +321,11 @@ inline uint32_t padded_side(uint32_t C, uint32_t T) { return C <= T ? T : (uint3
 
 }  // namespace
 
-int batch_device_impl(DeviceContext* ctx, uint32_t E, const uint32_t* Cs, const uint32_t* ms, const int32_t* pos,
+int batch_device_impl(DeviceContext* ctx, uint32_t E, const uint32_t* Cs, const uint32_t* ms, const void* pos_any,
                       uint32_t* paths, uint32_t* ranking, uint32_t flags, hipStream_t s, se_batch_timing* timing) {
     uint64_t pos_total = 0, d_total = 0, r_total = 0;
     uint32_t max_m = 0;
+    const bool pos16 = (flags & SE_BATCH_POS16) != 0;
     for (uint32_t e = 0; e < E; ++e) {
         if (Cs[e] > kMaxBatchC) {
             set_error("batched elections are limited to 1024 candidates; use se_closure_device for larger ones");
@@ -352,7 +355,9 @@ int batch_device_impl(DeviceContext* ctx, uint32_t E, const uint32_t* Cs, const 
         for (uint32_t e = 0; e < E; ++e) {
             const uint32_t C = Cs[e], m = ms[e];
             if (C) {
-                int rc = count_device_impl(ctx, pos + po, false, m, C, dbuf, C, s, nullptr, nullptr);
+                const void* pe = pos16 ? static_cast<const void*>(static_cast<const uint16_t*>(pos_any) + po)
+                                       : static_cast<const void*>(static_cast<const int32_t*>(pos_any) + po);
+                int rc = count_device_impl(ctx, pe, pos16, m, C, dbuf, C, s, nullptr, nullptr);
                 if (rc == SE_OK) rc = closure_device_impl(ctx, dbuf, C, C, paths + doff, C, 0, s, nullptr);
                 if (rc == SE_OK) rc = wins_device_impl(ctx, paths + doff, C, C, wins, s);
                 if (rc != SE_OK) return rc;
@@ -500,8 +505,12 @@ int batch_device_impl(DeviceContext* ctx, uint32_t E, const uint32_t* Cs, const 
             return SE_ERR_COPY;
         if (any_sliced && !hip_ok(hipMemsetAsync(dbuf, 0, d_total * 4, s), "Failed to clear device memory"))
             return SE_ERR_DEVICE;
-        if (!sorted_items.empty())
-            batch_count_kernel<<<(uint32_t)sorted_items.size(), kCThreads, 0, s>>>(pos, dbuf, dmeta, ditems);
+        if (!sorted_items.empty() && pos16)
+            batch_count_kernel<uint16_t><<<(uint32_t)sorted_items.size(), kCThreads, 0, s>>>(
+                static_cast<const uint16_t*>(pos_any), dbuf, dmeta, ditems);
+        else if (!sorted_items.empty())
+            batch_count_kernel<int32_t><<<(uint32_t)sorted_items.size(), kCThreads, 0, s>>>(
+                static_cast<const int32_t*>(pos_any), dbuf, dmeta, ditems);
         SE_CHECK_LAUNCH("batch counter launch failed");
         // work lists of the round-closed elections, built while the counter runs
         uint32_t max_rounds = 0;

@@ -201,6 +201,27 @@ int narrow_counts16(const uint32_t* d, uint32_t n, size_t ldd, uint16_t* out, hi
     return SE_OK;
 }
 
+// int32 positions -> uint16 (batches with SE_BATCH_POS16); *flag records a value outside
+// 0..65535 (its low 16 bits are stored).
+__global__ __launch_bounds__(256) void narrow_positions_kernel(const int32_t* __restrict__ src, size_t count,
+                                                               uint16_t* __restrict__ dst, uint32_t* __restrict__ flag) {
+    uint32_t bad = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t v = (uint32_t)src[i];
+        bad |= v >> 16;
+        dst[i] = (uint16_t)v;
+    }
+    if (bad) atomicOr(flag, 1u);
+}
+
+int narrow_positions16(const int32_t* src, size_t count, uint16_t* dst, uint32_t* flag, hipStream_t s) {
+    if (count == 0) return SE_OK;
+    const size_t blocks = std::min<size_t>(4096, (count + 255) / 256);
+    narrow_positions_kernel<<<(uint32_t)blocks, 256, 0, s>>>(src, count, dst, flag);
+    SE_CHECK_LAUNCH("position conversion launch failed");
+    return SE_OK;
+}
+
 // Clears a device word (a kernel, not hipMemsetAsync, so the sequence stays capturable).
 int clear_flag(uint32_t* flag, hipStream_t s) {
     zero_matrix_kernel<<<1, 64, 0, s>>>(flag, 1, 1);

@@ -103,3 +103,34 @@ def test_batch_rejects_oversize_election(gpu_lib):
     db = batch.DeviceBatch(w)
     with pytest.raises(RuntimeError, match="1024 candidates"):
         db.run()
+
+
+def test_batch_int32_and_uint16_positions_agree(gpu_lib, oracle_mod):
+    """SE_BATCH_POS16 (uint16 positions, converted on the device) and int32 positions give
+    the same results, batched and per launch; the conversion rejects values above 65535."""
+    from scalingelections_amd import _lib, batch, devmem
+
+    rng = np.random.default_rng(3)
+    C = np.array([5, 64, 100, 128, 129, 200, 300, 512, 700], np.uint32)
+    m = rng.integers(1, 3000, C.shape[0]).astype(np.uint32)
+    w = batch.make_workload(C, m, rng.uniform(0.5, 1.0, C.shape[0]), seed=3)
+    a, b = batch.DeviceBatch(w, pos16=True), batch.DeviceBatch(w, pos16=False)
+    for per_launch in (False, True):
+        a.run(per_launch=per_launch)
+        b.run(per_launch=per_launch)
+        ra, rb = a.raw(), b.raw()
+        assert np.array_equal(ra[0], rb[0]) and np.array_equal(ra[1], rb[1]), per_launch
+    paths, ranks = a.results()
+    for e in (0, 3, 7):
+        pos = w.election_positions(e)
+        d = oracle_mod.build_prefs([np.argsort(row, kind="stable") for row in pos])
+        assert np.array_equal(paths[e], oracle_mod.closure_tiled(d, 16)), e
+    bad = np.array([0, 1, 65535, 65536, 7], np.int32)
+    src = devmem.DeviceBuffer.from_array(bad)
+    dst = devmem.DeviceBuffer(bad.size * 2)
+    rc = _lib.lib().se_positions16_device(src.ptr, bad.size, dst.ptr, None)
+    assert rc == _lib.SE_ERR_INVALID and "0..65535" in _lib.last_error()
+    ok = np.array([0, 1, 65535, 1023], np.int32)
+    src2 = devmem.DeviceBuffer.from_array(ok)
+    _lib.check(_lib.lib().se_positions16_device(src2.ptr, ok.size, dst.ptr, None))
+    assert np.array_equal(dst.download(np.empty(ok.size, np.uint16)), ok.astype(np.uint16))
