@@ -150,9 +150,11 @@ int se_rank_device(const uint32_t *paths_dev, uint32_t n, size_t row_stride, uin
  * or uint16 with SE_BATCH_POS16.
  * paths_dev: concatenation of C_e x C_e uint32 strongest-path matrices (zero diagonal).
  * ranking_dev: concatenation of C_e candidate indices by descending wins, ties ascending.
- * Default strategy: two launches for the whole batch (counting work items, then one
- * workgroup per election for p0, closure, wins and ranking).  SE_BATCH_PER_LAUNCH runs
- * every election through the single-election path instead (for comparison). */
+ * Default strategy: one counting launch over work items for the whole batch, then one
+ * workgroup per election for p0, closure, wins and ranking; on the 16-bit paths the
+ * elections above 128 candidates share one pair of launches per round instead (panels,
+ * then tiles).  SE_BATCH_PER_LAUNCH runs every election through the single-election path
+ * instead (for comparison). */
 #define SE_BATCH_PER_LAUNCH 64u
 /* Close every batched election in one workgroup, also those above 128 candidates, which by
  * default share per-round launches (panels, then tiles) with the batch's other large

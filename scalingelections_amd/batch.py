@@ -6,8 +6,9 @@ The batch returns, per election, the strongest-path matrix and the ranking that
 (scaling_elections.py:51-296, scaling_elections.cu:863-989) return for that election alone.
 
 Strategies (same outputs):
-  * batched (default): two launches for the whole batch (counting work items; one
-    workgroup per election for p0, closure, wins and ranking);
+  * batched (default): one counting launch over work items for the whole batch, then one
+    workgroup per election for p0, closure, wins and ranking -- elections above 128
+    candidates instead share one pair of launches per round (panels, then tiles);
   * per-launch: every election through the single-election entry points, one after the
     other -- the baseline the batched form is measured against.
 

@@ -737,10 +737,10 @@ int closure_device_ex(DeviceContext* ctx, const void* d_any, bool d16, uint32_t 
     int rc;
     uint16_t* W16 = static_cast<uint16_t*>(W);
     uint32_t* W32 = static_cast<uint32_t*>(W);
-    // look-ahead: the 16-bit paths run the next panel inside the launch of the current
-    // round's remaining tiles (one stream); the 32-bit path overlaps it from a side stream,
-    // which pays once a round's independent work clearly outlasts the panel kernel.
-    // SE_FLAG_NO_LOOKAHEAD / SE_FLAG_LOOKAHEAD override the table.
+    // look-ahead (the table's mode, or a flag): the 16-bit paths run the next panel inside
+    // the launch of the current round's remaining tiles (one stream; mode 2 also drops
+    // the strip launch); the 32-bit path overlaps it from a side stream, only on request
+    // (SE_FLAG_LOOKAHEAD), since that stream stays with the context.
     if (flags & SE_FLAG_NO_LOOKAHEAD)
         la_mode = kLaOff;
     else if (flags & SE_FLAG_LOOKAHEAD_FUSED)
