@@ -81,7 +81,6 @@ void* scratch(DeviceContext* ctx, Slot s, size_t bytes) {
     if (ctx->ptr[s]) {
         // the previous contents are dead; make sure no queued kernel still uses them
         (void)hipStreamSynchronize(ctx->stream);
-        if (ctx->side) (void)hipStreamSynchronize(ctx->side);
         (void)hipFree(ctx->ptr[s]);
         ctx->ptr[s] = nullptr;
         ctx->cap[s] = 0;
@@ -119,13 +118,6 @@ DeviceContext* context_for(void* stream) {
 
 static void free_context_scratch(DeviceContext* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
-    if (ctx->side) {
-        (void)hipStreamSynchronize(ctx->side);
-        (void)hipStreamDestroy(ctx->side);
-        (void)hipEventDestroy(ctx->ev_strips);
-        (void)hipEventDestroy(ctx->ev_panel);
-        ctx->side = nullptr;
-    }
     for (int i = 0; i < kSlotCount; ++i) {
         if (ctx->ptr[i]) (void)hipFree(ctx->ptr[i]);
         ctx->ptr[i] = nullptr;

@@ -167,19 +167,26 @@ __global__ __launch_bounds__(256) void p0_write_kernel(const D* __restrict__ d, 
 // W[K][K] before then: the independent kernel of this round only uses it for pivot-row and
 // pivot-column cells, which it does not store.
 
-template <typename E, uint32_t T>
-__global__ __launch_bounds__(256) void pivot_panel_kernel(E* __restrict__ W, size_t ldw, uint32_t kb,
-                                                          uint32_t nb, uint32_t* __restrict__ pivot_out,
-                                                          const uint32_t* __restrict__ pivot_prev,
-                                                          uint32_t kb_prev) {
-    __shared__ uint32_t P[T][T + 1];
-    __shared__ uint32_t S[kMacro * (T + 1)];
-    __shared__ uint32_t rowbuf[2][T], colbuf[2][T];
-    const uint32_t t = threadIdx.x;
-    const bool row_panel = blockIdx.x < nb;
-    const uint32_t c0 = (row_panel ? blockIdx.x : blockIdx.x - nb) * kMacro;
+// One workgroup of the 32-bit panel of the pivot block at kb: index < nb is row chunk
+// `index`, the others column chunk index - nb.  Workgroup 0 commits the previous closed
+// pivot (when given) and publishes this one.  LDS: pivot_panel_lds_words<T>() words.
+template <uint32_t T>
+constexpr uint32_t pivot_panel_lds_words() { return T * (T + 1) + kMacro * (T + 1) + 4 * T; }
 
-    if (blockIdx.x == 0 && pivot_prev != nullptr) {
+template <typename E, uint32_t T>
+__device__ __forceinline__ void pivot_panel(E* __restrict__ W, size_t ldw, uint32_t kb, uint32_t nb,
+                                            uint32_t index, uint32_t* __restrict__ pivot_out,
+                                            const uint32_t* __restrict__ pivot_prev, uint32_t kb_prev,
+                                            uint32_t* lds) {
+    uint32_t(*P)[T + 1] = reinterpret_cast<uint32_t(*)[T + 1]>(lds);
+    uint32_t* S = lds + T * (T + 1);
+    uint32_t(*rowbuf)[T] = reinterpret_cast<uint32_t(*)[T]>(S + kMacro * (T + 1));
+    uint32_t(*colbuf)[T] = rowbuf + 2;
+    const uint32_t t = threadIdx.x;
+    const bool row_panel = index < nb;
+    const uint32_t c0 = (row_panel ? index : index - nb) * kMacro;
+
+    if (index == 0 && pivot_prev != nullptr) {
         for (uint32_t q = t; q < T * T; q += kThreads)
             W[(size_t)(kb_prev + q / T) * ldw + kb_prev + q % T] = (E)pivot_prev[q];
     }
@@ -189,12 +196,21 @@ __global__ __launch_bounds__(256) void pivot_panel_kernel(E* __restrict__ W, siz
     else
         load_col_chunk<E, T>(W, ldw, kb, c0, S);
     close_pivot<E, T>(W, ldw, kb, T, P, rowbuf, colbuf);
-    if (blockIdx.x == 0)
+    if (index == 0)
         for (uint32_t q = t; q < T * T; q += kThreads) pivot_out[q] = P[q / T][q % T];
     if (row_panel)
         row_chunk_product<E, T>(W, ldw, kb, c0, P, S);
     else
         col_chunk_product<E, T>(W, ldw, kb, c0, P, S);
+}
+
+template <typename E, uint32_t T>
+__global__ __launch_bounds__(256) void pivot_panel_kernel(E* __restrict__ W, size_t ldw, uint32_t kb,
+                                                          uint32_t nb, uint32_t* __restrict__ pivot_out,
+                                                          const uint32_t* __restrict__ pivot_prev,
+                                                          uint32_t kb_prev) {
+    __shared__ uint32_t lds[pivot_panel_lds_words<T>()];
+    pivot_panel<E, T>(W, ldw, kb, nb, blockIdx.x, pivot_out, pivot_prev, kb_prev, lds);
 }
 
 // Copies the last round's closed pivot tile into W.
@@ -245,6 +261,32 @@ __global__ __launch_bounds__(256) void indep_u32_kernel(uint32_t* __restrict__ W
     __shared__ __align__(16) uint32_t lds[indep_u32_lds_elems<T>()];
     uint32_t bi, bj;
     if (!select_tile(which, mnext, nb, &bi, &bj)) return;
+    indep_tile_u32<T>(W, ldw, kb, bi * kMacro, bj * kMacro, lds);
+}
+
+// 32-bit look-ahead launch, as panel_indep_u16_kernel: workgroups [0, npanel) are the panel
+// of the pivot block at kb_next (2 nb chunks; npanel is 2 nb rounded up to a multiple of
+// 8, the extra workgroups exit), the others the independent tiles of the pivot block at
+// kb outside macro row/column mnext (the strip launch did those).  The panel reads and
+// writes only macro row/column mnext and, in workgroup 0, commits the previous closed
+// pivot into the block at kb, which the tiles neither read nor store.
+template <uint32_t T>
+__global__ __launch_bounds__(256) void panel_indep_u32_kernel(uint32_t* __restrict__ W, size_t ldw, uint32_t kb,
+                                                              uint32_t mnext, uint32_t nb, uint32_t npanel,
+                                                              uint32_t kb_next, uint32_t* __restrict__ pivot_out,
+                                                              const uint32_t* __restrict__ pivot_prev) {
+    constexpr uint32_t kWords = indep_u32_lds_elems<T>() > pivot_panel_lds_words<T>() ? indep_u32_lds_elems<T>()
+                                                                                      : pivot_panel_lds_words<T>();
+    __shared__ __align__(16) uint32_t lds[kWords];
+    if (blockIdx.x < npanel) {
+        if (blockIdx.x >= 2 * nb) return;
+        __builtin_amdgcn_s_setprio(3);  // the round's critical path, as in the 16-bit launch
+        pivot_panel<uint32_t, T>(W, ldw, kb_next, nb, blockIdx.x, pivot_out, pivot_prev, kb, lds);
+        return;
+    }
+    const uint32_t t = blockIdx.x - npanel;
+    const uint32_t bi = t / nb, bj = t % nb;
+    if (bi == mnext || bj == mnext) return;  // strip tiles ran in the strip launch
     indep_tile_u32<T>(W, ldw, kb, bi * kMacro, bj * kMacro, lds);
 }
 
@@ -446,15 +488,6 @@ uint32_t init_grid(uint32_t nt) {
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(pairs, 4096));
 }
 
-bool ensure_side_stream(DeviceContext* ctx) {
-    if (ctx->side) return true;
-    if (!hip_ok(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking), "Failed to create a HIP stream") ||
-        !hip_ok(hipEventCreateWithFlags(&ctx->ev_strips, hipEventDisableTiming), "Failed to create events") ||
-        !hip_ok(hipEventCreateWithFlags(&ctx->ev_panel, hipEventDisableTiming), "Failed to create events"))
-        return false;
-    return true;
-}
-
 template <typename E, uint32_t T, bool F16>
 void launch_indep(E* W, size_t ldw, uint32_t kb, uint32_t which, uint32_t mnext, uint32_t nb, hipStream_t s) {
     const dim3 grid = which == 1 ? dim3(2 * nb - 1) : dim3(nb, nb);
@@ -464,14 +497,14 @@ void launch_indep(E* W, size_t ldw, uint32_t kb, uint32_t which, uint32_t mnext,
         indep_u32_kernel<T><<<grid, kThreads, 0, s>>>(W, ldw, kb, which, mnext, nb);
 }
 
-// Rounds of the blocked closure.  With look-ahead (default, >= 2 rounds): round r's
-// independent kernel is split into the tiles of the next pivot's strips, launched first,
-// and the rest; the next round's pivot/panel kernel runs on the side stream as soon as
-// the strips are done, overlapping the rest of round r.  Neither side touches what the
-// other writes: the rest kernel excludes the strip tiles and does not read the pivot block.
-template <typename E, uint32_t T, bool F16 = false>
-int run_rounds(DeviceContext* ctx, E* W, size_t ldw, uint32_t n, uint32_t N, uint32_t* pivot_buf, hipStream_t s,
-               EventTimer& tm, EventTimer& tm_side, bool lookahead, uint32_t* rounds_out) {
+// Rounds of the 32-bit closure, all on stream s.  With look-ahead (kLaStrips, >= 2
+// rounds) round r is two launches: the tiles of the next pivot's strips (macro row/column
+// mnext), then panel_indep_u32_kernel holding the next round's panel and the rest of round
+// r's tiles, so the panel overlaps the bulk of the round.  Without it (kLaOff): all
+// independent tiles, then the next panel.
+template <uint32_t T>
+int run_rounds32(uint32_t* W, size_t ldw, uint32_t n, uint32_t N, uint32_t* pivot_buf, hipStream_t s,
+                 EventTimer& tm, uint32_t la_mode, uint32_t* rounds_out) {
     const uint32_t nb = N / kMacro;
     const uint32_t rounds = (n + T - 1) / T;
     auto pivot_of = [&](uint32_t r) { return pivot_buf + (r & 1) * T * T; };
@@ -479,10 +512,9 @@ int run_rounds(DeviceContext* ctx, E* W, size_t ldw, uint32_t n, uint32_t N, uin
         *rounds_out = 0;
         return SE_OK;
     }
-    lookahead = lookahead && rounds >= 2;
-    if (lookahead && !ensure_side_stream(ctx)) return SE_ERR_DEVICE;
+    const uint32_t npanel = (2 * nb + 7) / 8 * 8;
     tm.mark(kKindPanel);
-    pivot_panel_kernel<E, T><<<2 * nb, kThreads, 0, s>>>(W, ldw, 0, nb, pivot_of(0), nullptr, 0);
+    pivot_panel_kernel<uint32_t, T><<<2 * nb, kThreads, 0, s>>>(W, ldw, 0, nb, pivot_of(0), nullptr, 0);
     tm.mark(kKindPanel);
     for (uint32_t r = 0; r < rounds; ++r) {
         if (g_interrupted.load(std::memory_order_relaxed)) {
@@ -491,43 +523,33 @@ int run_rounds(DeviceContext* ctx, E* W, size_t ldw, uint32_t n, uint32_t N, uin
         }
         const uint32_t kb = r * T;
         const bool next = r + 1 < rounds;
-        if (!next) {
+        if (!next || la_mode == kLaOff) {
             tm.mark(kKindIndep);
-            launch_indep<E, T, F16>(W, ldw, kb, 0, 0, nb, s);
+            launch_indep<uint32_t, T, false>(W, ldw, kb, 0, 0, nb, s);
             tm.mark(kKindIndep);
-        } else if (!lookahead) {
-            tm.mark(kKindIndep);
-            launch_indep<E, T, F16>(W, ldw, kb, 0, 0, nb, s);
-            tm.mark(kKindIndep);
-            tm.mark(kKindPanel);
-            pivot_panel_kernel<E, T><<<2 * nb, kThreads, 0, s>>>(W, ldw, kb + T, nb, pivot_of(r + 1), pivot_of(r), kb);
-            tm.mark(kKindPanel);
+            if (next) {
+                tm.mark(kKindPanel);
+                pivot_panel_kernel<uint32_t, T>
+                    <<<2 * nb, kThreads, 0, s>>>(W, ldw, kb + T, nb, pivot_of(r + 1), pivot_of(r), kb);
+                tm.mark(kKindPanel);
+            }
         } else {
             const uint32_t mnext = (kb + T) / kMacro;
             tm.mark(kKindIndep);
-            launch_indep<E, T, F16>(W, ldw, kb, 1, mnext, nb, s);
-            if (!hip_ok(hipEventRecord(ctx->ev_strips, s), "") ||
-                !hip_ok(hipStreamWaitEvent(ctx->side, ctx->ev_strips, 0), ""))
-                return SE_ERR_DEVICE;
-            tm_side.mark(kKindPanel);
-            pivot_panel_kernel<E, T>
-                <<<2 * nb, kThreads, 0, ctx->side>>>(W, ldw, kb + T, nb, pivot_of(r + 1), pivot_of(r), kb);
-            tm_side.mark(kKindPanel);
-            if (!hip_ok(hipEventRecord(ctx->ev_panel, ctx->side), "")) return SE_ERR_DEVICE;
-            launch_indep<E, T, F16>(W, ldw, kb, 2, mnext, nb, s);
+            launch_indep<uint32_t, T, false>(W, ldw, kb, 1, mnext, nb, s);
+            panel_indep_u32_kernel<T><<<npanel + nb * nb, kThreads, 0, s>>>(W, ldw, kb, mnext, nb, npanel, kb + T,
+                                                                             pivot_of(r + 1), pivot_of(r));
             tm.mark(kKindIndep);
-            if (!hip_ok(hipStreamWaitEvent(s, ctx->ev_panel, 0), "")) return SE_ERR_DEVICE;
         }
         SE_CHECK_LAUNCH("closure round launch failed");
     }
     tm.mark(kKindPanel);
-    commit_pivot_kernel<E, T><<<1, kThreads, 0, s>>>(W, ldw, (rounds - 1) * T, pivot_of(rounds - 1));
+    commit_pivot_kernel<uint32_t, T><<<1, kThreads, 0, s>>>(W, ldw, (rounds - 1) * T, pivot_of(rounds - 1));
     tm.mark(kKindPanel);
     SE_CHECK_LAUNCH("closure round launch failed");
     *rounds_out = rounds;
     return SE_OK;
 }
-
 
 // Rounds of the 16-bit closure, by look-ahead mode:
 //   kLaStrips: round r is two launches, the strip tiles of the next pivot (macro
@@ -737,10 +759,9 @@ int closure_device_ex(DeviceContext* ctx, const void* d_any, bool d16, uint32_t 
     int rc;
     uint16_t* W16 = static_cast<uint16_t*>(W);
     uint32_t* W32 = static_cast<uint32_t*>(W);
-    // look-ahead (the table's mode, or a flag): the 16-bit paths run the next panel inside
-    // the launch of the current round's remaining tiles (one stream; mode 2 also drops
-    // the strip launch); the 32-bit path overlaps it from a side stream, only on request
-    // (SE_FLAG_LOOKAHEAD), since that stream stays with the context.
+    // look-ahead (the table's mode, or a flag): the next panel runs inside the launch of
+    // the current round's remaining tiles, on the one stream s (16-bit mode 2 also drops
+    // the strip launch).
     if (flags & SE_FLAG_NO_LOOKAHEAD)
         la_mode = kLaOff;
     else if (flags & SE_FLAG_LOOKAHEAD_FUSED)
@@ -748,9 +769,6 @@ int closure_device_ex(DeviceContext* ctx, const void* d_any, bool d16, uint32_t 
     else if (flags & SE_FLAG_LOOKAHEAD)
         la_mode = kLaStrips;
     if (la_mode == kLaFused && (!use16 || T != 64)) la_mode = kLaStrips;  // 16-bit, pivot 64 only
-    const bool la32 = !use16 && la_mode != kLaOff;
-    if (la32 && !ensure_side_stream(ctx)) return SE_ERR_DEVICE;
-    EventTimer tm_side((flags & SE_FLAG_TIME_KERNELS) != 0 && la32, ctx->side);
     if (use16) {
         if (f16)
             rc = T == 64 ? run_rounds16<64, true>(W16, ldw, n, N, pivot_buf, s, tm, la_mode, &rounds)
@@ -759,8 +777,8 @@ int closure_device_ex(DeviceContext* ctx, const void* d_any, bool d16, uint32_t 
             rc = T == 64 ? run_rounds16<64, false>(W16, ldw, n, N, pivot_buf, s, tm, la_mode, &rounds)
                          : run_rounds16<32, false>(W16, ldw, n, N, pivot_buf, s, tm, la_mode, &rounds);
     } else {
-        rc = T == 64 ? run_rounds<uint32_t, 64>(ctx, W32, ldw, n, N, pivot_buf, s, tm, tm_side, la32, &rounds)
-                     : run_rounds<uint32_t, 32>(ctx, W32, ldw, n, N, pivot_buf, s, tm, tm_side, la32, &rounds);
+        rc = T == 64 ? run_rounds32<64>(W32, ldw, n, N, pivot_buf, s, tm, la_mode, &rounds)
+                     : run_rounds32<32>(W32, ldw, n, N, pivot_buf, s, tm, la_mode, &rounds);
     }
     if (rc != SE_OK) return rc;
 
@@ -789,7 +807,6 @@ int closure_device_ex(DeviceContext* ctx, const void* d_any, bool d16, uint32_t 
         t.total_ms = ms;
         double sums[5] = {0, 0, 0, 0, 0};
         tm.collect(sums);
-        tm_side.collect(sums);
         t.init_ms = sums[kKindInit];
         t.diag_ms = sums[kKindDiag];
         t.panel_ms = sums[kKindPanel];

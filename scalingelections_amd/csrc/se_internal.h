@@ -43,8 +43,6 @@ struct DeviceContext {
     int device = -1;
     hipStream_t stream = nullptr;
     bool owns_stream = true;
-    hipStream_t side = nullptr;   // look-ahead pivot/panel stream (created on first use)
-    hipEvent_t ev_strips = nullptr, ev_panel = nullptr;
     std::mutex lock;
     void* ptr[kSlotCount] = {};
     size_t cap[kSlotCount] = {};

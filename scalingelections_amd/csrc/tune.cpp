@@ -23,16 +23,14 @@ namespace {
 // 3.03 ms); the one-launch-per-round look-ahead wins from 6144 on (8192 f16: 13.56 vs
 // 14.24 ms), where a round's tiles outlast the panel workgroups' extra strip update, and
 // loses below (4096: 2.83 vs 2.64 ms).  The 32-bit path prefers pivot 32 up to 1024 and
-// pivot 64 above (8192: 31.2 vs 37.2 ms).  Its look-ahead (a side stream) gains 2% from
-// 16384 on (217 vs 222 ms) but is off here: the side stream stays with the context, takes
-// one of the process's hardware queues, and the streams of other callers then share
-// queues (serving latency p50 11 -> 22-28 ms after one such closure, measured);
-// SE_FLAG_LOOKAHEAD still asks for it.
+// pivot 64 above (8192: 31.2 vs 37.2 ms); its look-ahead (strip launch, then the next
+// panel inside the launch of the round's other tiles) gains 1.4-5% from 2048 on (2048:
+// 1.68 vs 1.76 ms; 8192: 30.6 vs 31.4 ms; 32768: 1689 vs 1713 ms).
 const std::vector<se_tune_entry>& builtin_table() {
     static const std::vector<se_tune_entry> t = {
         {SE_TUNE_F16, 2048, 32, 1},       {SE_TUNE_F16, 4096, 64, 1},       {SE_TUNE_F16, 0xFFFFFFFFu, 64, 2},
         {SE_TUNE_U16, 2048, 32, 1},       {SE_TUNE_U16, 4096, 64, 1},       {SE_TUNE_U16, 0xFFFFFFFFu, 64, 2},
-        {SE_TUNE_U32, 1024, 32, 0},       {SE_TUNE_U32, 0xFFFFFFFFu, 64, 0},
+        {SE_TUNE_U32, 1024, 32, 0},       {SE_TUNE_U32, 0xFFFFFFFFu, 64, 1},
     };
     return t;
 }

@@ -47,7 +47,7 @@ def test_c2_counter_closure_ranking(gpu_lib):
     assert _sha(order.astype(np.int64), "<i8") == fx["ranking_sha256"]
 
 
-@pytest.mark.parametrize("flags", ["auto", "strips", "u32"])
+@pytest.mark.parametrize("flags", ["auto", "strips", "u32", "u32-strips"])
 def test_c3_32768_closure(gpu_lib, oracle_mod, flags):
     from scalingelections_amd import _lib, strongest_paths_timed, synth
 
@@ -57,10 +57,11 @@ def test_c3_32768_closure(gpu_lib, oracle_mod, flags):
     d = synth.synthesized_preferences(n, seed)
     assert _sha(d) == fx["d_sha256"]
     # auto: the tuned schedule (one launch per round); strips: the next pivot's strips in a
-    # launch of their own
-    f = {"auto": 0, "strips": _lib.SE_FLAG_LOOKAHEAD, "u32": _lib.SE_FLAG_FORCE_U32}[flags]
+    # launch of their own, then the next panel inside the launch of the round's other tiles
+    f = {"auto": 0, "strips": _lib.SE_FLAG_LOOKAHEAD, "u32": _lib.SE_FLAG_FORCE_U32,
+         "u32-strips": _lib.SE_FLAG_FORCE_U32 | _lib.SE_FLAG_LOOKAHEAD}[flags]
     p, t = strongest_paths_timed(d, f)
-    assert t["width_bits"] == (32 if flags == "u32" else 16)
+    assert t["width_bits"] == (32 if flags.startswith("u32") else 16)
     assert _sha(p) == fx["p_sha256"]
     if flags == "auto":
         from scalingelections_amd import wins_and_order
