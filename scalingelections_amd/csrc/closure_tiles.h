@@ -509,37 +509,81 @@ __device__ __forceinline__ void indep_tile64_u16(uint16_t* __restrict__ W, size_
 template <uint32_t T>
 constexpr uint32_t indep_u32_lds_elems() { return kMacro * (T + 4) + T * kMacro; }
 
-template <uint32_t T>
-__device__ __forceinline__ void indep_tile_u32(uint32_t* __restrict__ W, size_t ldw, uint32_t kb, uint32_t I0,
-                                               uint32_t J0, uint32_t* __restrict__ lds) {
+// LATE (pivot 64, where LDS already limits the kernel to 2 waves per SIMD and 256 VGPRs):
+// strips loaded first, C loaded behind them and merged after the product, as in
+// indep_tile_u16_impl; with pivot 32 (4 waves, 128 VGPRs) C seeds the accumulators.
+// MASKED = false: no pivot row or column in the tile, so no per-row checks (pivot 64).
+template <uint32_t T, bool MASKED>
+__device__ __forceinline__ void indep_tile_u32_impl(uint32_t* __restrict__ W, size_t ldw, uint32_t kb,
+                                                    uint32_t I0, uint32_t J0, uint32_t* __restrict__ lds) {
+    constexpr bool LATE = T == 64;
     constexpr uint32_t AP = T + 4;  // A strip pitch (u32)
     uint32_t* As = lds;
     uint32_t* Bs = lds + kMacro * AP;
     const uint32_t t = threadIdx.x, tx = t & 15, ty = t >> 4;
     __syncthreads();
 
-    uint4 c[8][2];  // C tile without its pivot rows/columns (see indep_tile_u16)
+    // C tile without its pivot rows/columns (see indep_tile_u16_impl)
+    uint4 c[8][2];
     const uint32_t* cp = W + (size_t)(I0 + ty) * ldw + J0 + tx * 4;
-    const bool lo_in_pivot = J0 + tx * 4 - kb < T, hi_in_pivot = J0 + 64 + tx * 4 - kb < T;
+    const bool lo_in_pivot = MASKED && J0 + tx * 4 - kb < T, hi_in_pivot = MASKED && J0 + 64 + tx * 4 - kb < T;
+    auto load_c = [&]() {
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-        const bool row_ok = I0 + ty + 16 * r - kb >= T;
-        c[r][0] = make_uint4(0, 0, 0, 0);
-        c[r][1] = make_uint4(0, 0, 0, 0);
-        if (row_ok && !lo_in_pivot) c[r][0] = *reinterpret_cast<const uint4*>(cp + (size_t)(16 * r) * ldw);
-        if (row_ok && !hi_in_pivot) c[r][1] = *reinterpret_cast<const uint4*>(cp + (size_t)(16 * r) * ldw + 64);
-    }
-    for (uint32_t q = t; q < kMacro * (T / 4); q += kThreads) {  // strips, pivot block as zeros
-        uint32_t row = q / (T / 4), ch = q % (T / 4);
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (I0 + row - kb >= T) v = *reinterpret_cast<const uint4*>(W + (size_t)(I0 + row) * ldw + kb + ch * 4);
-        *reinterpret_cast<uint4*>(As + row * AP + ch * 4) = v;
-    }
-    for (uint32_t q = t; q < T * 32; q += kThreads) {
-        uint32_t row = q >> 5, ch = q & 31;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (J0 + ch * 4 - kb >= T) v = *reinterpret_cast<const uint4*>(W + (size_t)(kb + row) * ldw + J0 + ch * 4);
-        *reinterpret_cast<uint4*>(Bs + row * kMacro + ch * 4) = v;
+        for (int r = 0; r < 8; ++r) {
+            const bool row_ok = !MASKED || I0 + ty + 16 * r - kb >= T;
+            c[r][0] = make_uint4(0, 0, 0, 0);
+            c[r][1] = make_uint4(0, 0, 0, 0);
+            if (row_ok && !lo_in_pivot) c[r][0] = *reinterpret_cast<const uint4*>(cp + (size_t)(16 * r) * ldw);
+            if (row_ok && !hi_in_pivot) c[r][1] = *reinterpret_cast<const uint4*>(cp + (size_t)(16 * r) * ldw + 64);
+        }
+    };
+    if constexpr (LATE) {
+        // strips through registers, pivot block as zeros; C behind them
+        constexpr uint32_t kAq = kMacro * (T / 4) / kThreads, kBq = T * 32 / kThreads;
+        static_assert(kMacro * (T / 4) % kThreads == 0 && T * 32 % kThreads == 0, "strip chunks per lane");
+        uint4 sa[kAq], sb[kBq];
+#pragma unroll
+        for (uint32_t i = 0; i < kAq; ++i) {
+            const uint32_t q = t + i * kThreads, row = q / (T / 4), ch = q % (T / 4);
+            sa[i] = make_uint4(0, 0, 0, 0);
+            if (!MASKED || I0 + row - kb >= T)
+                sa[i] = *reinterpret_cast<const uint4*>(W + (size_t)(I0 + row) * ldw + kb + ch * 4);
+        }
+#pragma unroll
+        for (uint32_t i = 0; i < kBq; ++i) {
+            const uint32_t q = t + i * kThreads, row = q >> 5, ch = q & 31;
+            sb[i] = make_uint4(0, 0, 0, 0);
+            if (!MASKED || J0 + ch * 4 - kb >= T)
+                sb[i] = *reinterpret_cast<const uint4*>(W + (size_t)(kb + row) * ldw + J0 + ch * 4);
+        }
+        load_c();
+#pragma unroll
+        for (uint32_t i = 0; i < kAq; ++i) {
+            const uint32_t q = t + i * kThreads;
+            *reinterpret_cast<uint4*>(As + (q / (T / 4)) * AP + (q % (T / 4)) * 4) = sa[i];
+        }
+#pragma unroll
+        for (uint32_t i = 0; i < kBq; ++i) {
+            const uint32_t q = t + i * kThreads;
+            *reinterpret_cast<uint4*>(Bs + (q >> 5) * kMacro + (q & 31) * 4) = sb[i];
+        }
+    } else {
+        // C first, then the strips (pivot block as zeros) straight into LDS
+        load_c();
+        for (uint32_t q = t; q < kMacro * (T / 4); q += kThreads) {
+            const uint32_t row = q / (T / 4), ch = q % (T / 4);
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (!MASKED || I0 + row - kb >= T)
+                v = *reinterpret_cast<const uint4*>(W + (size_t)(I0 + row) * ldw + kb + ch * 4);
+            *reinterpret_cast<uint4*>(As + row * AP + ch * 4) = v;
+        }
+        for (uint32_t q = t; q < T * 32; q += kThreads) {
+            const uint32_t row = q >> 5, ch = q & 31;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (!MASKED || J0 + ch * 4 - kb >= T)
+                v = *reinterpret_cast<const uint4*>(W + (size_t)(kb + row) * ldw + J0 + ch * 4);
+            *reinterpret_cast<uint4*>(Bs + row * kMacro + ch * 4) = v;
+        }
     }
     __syncthreads();
 
@@ -548,8 +592,8 @@ __device__ __forceinline__ void indep_tile_u32(uint32_t* __restrict__ W, size_t 
     for (int r = 0; r < 8; ++r)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            acc[r][q] = word(c[r][0], q);
-            acc[r][4 + q] = word(c[r][1], q);
+            acc[r][q] = LATE ? 0u : word(c[r][0], q);
+            acc[r][4 + q] = LATE ? 0u : word(c[r][1], q);
         }
 
 #pragma unroll 1
@@ -577,19 +621,38 @@ __device__ __forceinline__ void indep_tile_u32(uint32_t* __restrict__ W, size_t 
             }
         }
     }
+    if constexpr (LATE) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                acc[r][q] = max(acc[r][q], word(c[r][0], q));
+                acc[r][4 + q] = max(acc[r][4 + q], word(c[r][1], q));
+            }
+    }
 
-    const bool lo_pivot = J0 + tx * 4 - kb < T, hi_pivot = J0 + 64 + tx * 4 - kb < T;
     uint32_t* op = W + (size_t)(I0 + ty) * ldw + J0 + tx * 4;
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
-        if (I0 + ty + 16 * r - kb < T) continue;
-        if (!lo_pivot)
+        if (MASKED && I0 + ty + 16 * r - kb < T) continue;
+        if (!lo_in_pivot)
             *reinterpret_cast<uint4*>(op + (size_t)(16 * r) * ldw) =
                 make_uint4(acc[r][0], acc[r][1], acc[r][2], acc[r][3]);
-        if (!hi_pivot)
+        if (!hi_in_pivot)
             *reinterpret_cast<uint4*>(op + (size_t)(16 * r) * ldw + 64) =
                 make_uint4(acc[r][4], acc[r][5], acc[r][6], acc[r][7]);
     }
+}
+
+template <uint32_t T>
+__device__ __forceinline__ void indep_tile_u32(uint32_t* __restrict__ W, size_t ldw, uint32_t kb, uint32_t I0,
+                                               uint32_t J0, uint32_t* __restrict__ lds) {
+    // pivot 32 keeps the checks everywhere: unconditional strip loads would be hoisted
+    // together, past the 128 VGPRs of 4 waves per SIMD
+    if (T == 64 && (I0 + kMacro <= kb || I0 >= kb + T) && (J0 + kMacro <= kb || J0 >= kb + T))
+        indep_tile_u32_impl<T, false>(W, ldw, kb, I0, J0, lds);
+    else
+        indep_tile_u32_impl<T, true>(W, ldw, kb, I0, J0, lds);
 }
 
 // ---------------------------------------------------------------------------------------
