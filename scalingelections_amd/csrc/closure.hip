@@ -38,6 +38,9 @@ namespace se {
 // over the upper triangle of the nt x nt tile grid and visited grid-stride.
 
 constexpr uint32_t kInitTile = 64;
+// The p0 range (max, min of the nonzero p0 values) is reduced into kRangeSlots pairs,
+// kRangePitch words apart, that the host combines.
+constexpr uint32_t kRangeSlots = 32, kRangePitch = 16;
 
 // Pair t -> (I, J), I <= J, of the row-major upper triangle of an nt x nt grid.
 __device__ __forceinline__ void upper_pair(uint32_t t, uint32_t nt, uint32_t* I, uint32_t* J) {
@@ -112,8 +115,11 @@ __global__ __launch_bounds__(256) void p0_max_kernel(const D* __restrict__ d, ui
     if (threadIdx.x == 0) {
         uint32_t r = max(max(red[0], red[1]), max(red[2], red[3]));
         uint32_t rn = min(min(redmin[0], redmin[1]), min(redmin[2], redmin[3]));
-        if (r) atomicMax(out, r);
-        if (rn != 0xFFFFFFFFu) atomicMin(out + 1, rn);
+        // one of kRangeSlots (max, min) pairs, a 64-byte line each: same-address atomics
+        // from thousands of workgroups serialize (4096 candidates: 60 us with one pair)
+        uint32_t* slot = out + (blockIdx.x % kRangeSlots) * kRangePitch;
+        if (r) atomicMax(slot, r);
+        if (rn != 0xFFFFFFFFu) atomicMin(slot + 1, rn);
     }
 }
 
@@ -436,9 +442,9 @@ __global__ __launch_bounds__(256) void output_kernel(const E* __restrict__ W, si
 }
 
 __global__ void init_range_kernel(uint32_t* p) {
-    if (threadIdx.x == 0) {
-        p[0] = 0u;
-        p[1] = 0xFFFFFFFFu;
+    if (threadIdx.x < kRangeSlots) {
+        p[threadIdx.x * kRangePitch] = 0u;
+        p[threadIdx.x * kRangePitch + 1] = 0xFFFFFFFFu;
     }
 }
 
@@ -696,9 +702,9 @@ int closure_device_ex(DeviceContext* ctx, const void* d_any, bool d16, uint32_t 
     if (flags & SE_FLAG_FORCE_U32) {
         use16 = false;
     } else {
-        uint32_t* dmax = static_cast<uint32_t*>(scratch(ctx, kSlotSmall, 256));
+        uint32_t* dmax = static_cast<uint32_t*>(scratch(ctx, kSlotSmall, kRangeSlots * kRangePitch * 4));
         if (!dmax) return SE_ERR_ALLOC;
-        uint32_t range[2] = {0, 0};
+        uint32_t slots[kRangeSlots * kRangePitch];
         tm.mark(kKindInit);
         init_range_kernel<<<1, 64, 0, s>>>(dmax);
         const uint32_t ntn = (n + kInitTile - 1) / kInitTile;
@@ -708,10 +714,15 @@ int closure_device_ex(DeviceContext* ctx, const void* d_any, bool d16, uint32_t 
             p0_max_kernel<uint32_t><<<init_grid(ntn), kThreads, 0, s>>>(d32, n, ld, ntn, dmax);
         tm.mark(kKindInit);
         SE_CHECK_LAUNCH("p0 init launch failed");
-        if (!hip_ok(hipMemcpyAsync(range, dmax, sizeof(range), hipMemcpyDeviceToHost, s),
+        if (!hip_ok(hipMemcpyAsync(slots, dmax, sizeof(slots), hipMemcpyDeviceToHost, s),
                     "Failed to copy data from device to host") ||
             !hip_ok(hipStreamSynchronize(s), "CUDA operations did not complete successfully"))
             return SE_ERR_COPY;
+        uint32_t range[2] = {0u, 0xFFFFFFFFu};
+        for (uint32_t q = 0; q < kRangeSlots; ++q) {
+            range[0] = std::max(range[0], slots[q * kRangePitch]);
+            range[1] = std::min(range[1], slots[q * kRangePitch + 1]);
+        }
         use16 = range[0] <= 0xFFFFu;
         if ((flags & SE_FLAG_FORCE_U16) && !use16) {
             set_error("strongest-path values exceed 65535; the 16-bit closure is not exact here");
