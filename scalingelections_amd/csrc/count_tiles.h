@@ -177,14 +177,18 @@ __device__ __forceinline__ void stage_words(const Pos* __restrict__ pos, size_t 
     const uint32_t guard = L.side ? 0u : PackTraits<K>::guard;
     const Pos* col = pos + base + L.c;
     const uint32_t first = SPLIT ? L.sub : 0u, step = SPLIT ? L.G : 1u;
-#pragma unroll 4
-    for (uint32_t w = first; w < kVW; w += step) {
-        uint32_t word = 0;
+    // Q words per step with every load unconditional (voters past the slice read voter
+    // ve - 1, in bounds since ve > vb, and count as 0), so the loads of a step issue back to
+    // back; a guarded load per voter made every word wait for its own round trip.  Split
+    // staging ends with its last (< Q) words one at a time.
+    auto packed = [&](uint32_t w) {
         const uint32_t v0 = vb + (w0 + w) * K;
+        uint32_t word = 0;
 #pragma unroll
         for (uint32_t k = 0; k < K; ++k) {
             const uint32_t v = v0 + k;
-            const uint32_t p = v < ve ? (uint32_t)col[(size_t)v * ldp] : 0u;
+            const uint32_t raw = (uint32_t)col[(size_t)min(v, ve - 1) * ldp];
+            const uint32_t p = v < ve ? raw : 0u;
             if constexpr (K > 1) {
                 word |= p << (PackTraits<K>::shift * k);
                 s += p;
@@ -193,8 +197,21 @@ __device__ __forceinline__ void stage_words(const Pos* __restrict__ pos, size_t 
                 word = p;
             }
         }
-        P[w][L.c] = word | guard;
+        return word;
+    };
+    // words per step: 4, or 2 for bytes in split staging (16 loads in flight did not fit
+    // beside the split kernels' accumulators and were serialized through one register)
+    constexpr uint32_t Q = (SPLIT && K == 4) ? 2u : 4u;
+    uint32_t w = first;
+    for (; w + (Q - 1) * step < kVW; w += Q * step) {
+        uint32_t wd[Q];
+#pragma unroll
+        for (uint32_t u = 0; u < Q; ++u) wd[u] = packed(w + u * step);
+#pragma unroll
+        for (uint32_t u = 0; u < Q; ++u) P[w + u * step][L.c] = wd[u] | guard;
     }
+    if constexpr (SPLIT)
+        for (; w < kVW; w += step) P[w][L.c] = packed(w) | guard;
 }
 
 // Per-candidate position sums from the threads' shares: sums[side * 128 + c] (256 words of
