@@ -58,6 +58,7 @@ struct PackTraits<1> {
 };
 
 typedef short short2_t __attribute__((ext_vector_type(2)));
+typedef unsigned short ushort2_t __attribute__((ext_vector_type(2)));
 
 // One K-packed word of row side x against column side y.
 template <uint32_t K>
@@ -177,41 +178,61 @@ __device__ __forceinline__ void stage_words(const Pos* __restrict__ pos, size_t 
     const uint32_t guard = L.side ? 0u : PackTraits<K>::guard;
     const Pos* col = pos + base + L.c;
     const uint32_t first = SPLIT ? L.sub : 0u, step = SPLIT ? L.G : 1u;
-    // Q words per step with every load unconditional (voters past the slice read voter
-    // ve - 1, in bounds since ve > vb, and count as 0), so the loads of a step issue back to
-    // back; a guarded load per voter made every word wait for its own round trip.  Split
-    // staging ends with its last (< Q) words one at a time.
-    auto packed = [&](uint32_t w) {
+    // Q words per step with every load unconditional, so the loads of a step issue back to
+    // back (a guarded load per voter made every word wait for its own round trip).  A round
+    // wholly inside the slice (every round but the last) reads straight through; the last
+    // one reads voter ve - 1 (in bounds: ve > vb) for voters past the slice and counts them
+    // as 0.  The position sum takes one unsigned dot product per packed word; the range
+    // check ORs every unpacked position.  Split staging ends with its last (< Q) words one at
+    // a time.
+    const bool whole = vb + (w0 + kVW) * K <= ve;  // uniform over the workgroup
+    auto packed = [&](uint32_t w, auto whole_tag) {
+        constexpr bool WHOLE = decltype(whole_tag)::value;
         const uint32_t v0 = vb + (w0 + w) * K;
+        const Pos* pv = col + (size_t)v0 * ldp;
         uint32_t word = 0;
 #pragma unroll
         for (uint32_t k = 0; k < K; ++k) {
-            const uint32_t v = v0 + k;
-            const uint32_t raw = (uint32_t)col[(size_t)min(v, ve - 1) * ldp];
-            const uint32_t p = v < ve ? raw : 0u;
+            uint32_t p;
+            if constexpr (WHOLE) {
+                p = (uint32_t)pv[(size_t)k * ldp];
+            } else {
+                const uint32_t v = v0 + k;
+                const uint32_t raw = (uint32_t)col[(size_t)min(v, ve - 1) * ldp];
+                p = v < ve ? raw : 0u;
+            }
             if constexpr (K > 1) {
                 word |= p << (PackTraits<K>::shift * k);
-                s += p;
                 bits |= p;
             } else {
                 word = p;
             }
         }
+        if constexpr (K == 4)
+            s = __builtin_amdgcn_udot4(word, 0x01010101u, s, false);
+        else if constexpr (K == 2)
+            s = __builtin_amdgcn_udot2(__builtin_bit_cast(ushort2_t, word), ushort2_t{1, 1}, s, false);
         return word;
     };
     // words per step: 4, or 2 for bytes in split staging (16 loads in flight did not fit
     // beside the split kernels' accumulators and were serialized through one register)
     constexpr uint32_t Q = (SPLIT && K == 4) ? 2u : 4u;
-    uint32_t w = first;
-    for (; w + (Q - 1) * step < kVW; w += Q * step) {
-        uint32_t wd[Q];
+    auto run = [&](auto whole_tag) {
+        uint32_t w = first;
+        for (; w + (Q - 1) * step < kVW; w += Q * step) {
+            uint32_t wd[Q];
 #pragma unroll
-        for (uint32_t u = 0; u < Q; ++u) wd[u] = packed(w + u * step);
+            for (uint32_t u = 0; u < Q; ++u) wd[u] = packed(w + u * step, whole_tag);
 #pragma unroll
-        for (uint32_t u = 0; u < Q; ++u) P[w + u * step][L.c] = wd[u] | guard;
-    }
-    if constexpr (SPLIT)
-        for (; w < kVW; w += step) P[w][L.c] = packed(w) | guard;
+            for (uint32_t u = 0; u < Q; ++u) P[w + u * step][L.c] = wd[u] | guard;
+        }
+        if constexpr (SPLIT)
+            for (; w < kVW; w += step) P[w][L.c] = packed(w, whole_tag) | guard;
+    };
+    if (whole)
+        run(std::true_type{});
+    else
+        run(std::false_type{});
 }
 
 // Per-candidate position sums from the threads' shares: sums[side * 128 + c] (256 words of
