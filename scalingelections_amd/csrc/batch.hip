@@ -134,7 +134,8 @@ constexpr uint32_t kPT = 64, kPP = kPT + 1;
 // p0 (scaling_elections.py:103-112) of one election (d: C x C) into its padded work matrix
 // (Cp x Cp): v = d[i][j] + bias when d[i][j] > d[j][i], else 0 (diagonal and padding 0).
 // A, B: 2 x 64 x 65 u32 of LDS.
-template <typename E>
+// U: rows staged per unrolled step (loads in flight per thread: 2 U).
+template <typename E, uint32_t U>
 __device__ void batch_p0(const uint32_t* __restrict__ d, uint32_t C, E* __restrict__ W, uint32_t Cp,
                          uint32_t bias, uint32_t* A, uint32_t* B) {
     const uint32_t t = threadIdx.x, c = t & (kPT - 1), r0 = t / kPT;
@@ -142,10 +143,15 @@ __device__ void batch_p0(const uint32_t* __restrict__ d, uint32_t C, E* __restri
     for (uint32_t I = 0; I < nt; ++I)
         for (uint32_t J = I; J < nt; ++J) {
             __syncthreads();
+            // unconditional loads at clamped indices (cells outside C x C count as 0), so
+            // the loads of U rows issue together instead of one round trip each
+#pragma unroll U
             for (uint32_t r = r0; r < kPT; r += kThreads / kPT) {
                 const uint32_t i = I * kPT + r, j = J * kPT + c, i2 = J * kPT + r, j2 = I * kPT + c;
-                A[r * kPP + c] = (i < C && j < C) ? d[(size_t)i * C + j] : 0u;
-                B[r * kPP + c] = (i2 < C && j2 < C) ? d[(size_t)i2 * C + j2] : 0u;
+                const uint32_t x = d[(size_t)min(i, C - 1) * C + min(j, C - 1)];
+                const uint32_t y = d[(size_t)min(i2, C - 1) * C + min(j2, C - 1)];
+                A[r * kPP + c] = (i < C && j < C) ? x : 0u;
+                B[r * kPP + c] = (i2 < C && j2 < C) ? y : 0u;
             }
             __syncthreads();
             for (uint32_t r = r0; r < kPT; r += kThreads / kPT) {
@@ -168,7 +174,7 @@ __device__ void batch_p0(const uint32_t* __restrict__ d, uint32_t C, E* __restri
 // (scaling_elections.py:287-290; the offset keeps the order) and the stable ranking by
 // descending wins, ties in ascending index (:292): rank(i) = #{j : w_j > w_i or (w_j = w_i
 // and j < i)}.  wins: C words of LDS; A, B as in batch_p0.
-template <typename E>
+template <typename E, uint32_t U>
 __device__ void batch_tail(const E* __restrict__ W, uint32_t Cp, uint32_t C, uint32_t bias, uint32_t* __restrict__ p,
                            uint32_t* __restrict__ rank_out, uint32_t* wins, uint32_t* A, uint32_t* B) {
     const uint32_t t = threadIdx.x, c = t & (kPT - 1), r0 = t / kPT;
@@ -177,10 +183,15 @@ __device__ void batch_tail(const E* __restrict__ W, uint32_t Cp, uint32_t C, uin
     for (uint32_t I = 0; I < nt; ++I)
         for (uint32_t J = I; J < nt; ++J) {
             __syncthreads();
+            // unconditional loads at clamped indices (cells outside C x C count as 0), so the
+            // loads of U rows issue together
+#pragma unroll U
             for (uint32_t r = r0; r < kPT; r += kThreads / kPT) {
                 const uint32_t i = I * kPT + r, j = J * kPT + c, i2 = J * kPT + r, j2 = I * kPT + c;
-                A[r * kPP + c] = (i < C && j < C) ? (uint32_t)W[(size_t)i * Cp + j] : 0u;
-                B[r * kPP + c] = (i2 < C && j2 < C) ? (uint32_t)W[(size_t)i2 * Cp + j2] : 0u;
+                const uint32_t x = W[(size_t)min(i, C - 1) * Cp + min(j, C - 1)];
+                const uint32_t y = W[(size_t)min(i2, C - 1) * Cp + min(j2, C - 1)];
+                A[r * kPP + c] = (i < C && j < C) ? x : 0u;
+                B[r * kPP + c] = (i2 < C && j2 < C) ? y : 0u;
             }
             __syncthreads();
             uint32_t beats = 0;  // column j = J * 64 + c over this tile's rows (I != J)
@@ -219,15 +230,14 @@ __device__ void batch_tail(const E* __restrict__ W, uint32_t Cp, uint32_t C, uin
 constexpr uint32_t kBatchF16Bias = 0x3FFu;
 constexpr uint32_t kBatchF16MaxVoters = 0x7C00u - 0x0400u;
 
-// phase 0: every election writes p0 into its work matrix; elections closed alone (piv ==
-// kNoRounds) also run the closure and the tail.  phase 1: the tail (output, wins, ranking)
-// of the round-closed elections.
+// The elections closed alone (piv == kNoRounds), one workgroup each: p0 into the work
+// matrix, the closure, then the tail (output, wins, ranking).  The round-closed elections
+// exit at once (batch_edge_kernel runs their p0 and tail).
 template <typename E, uint32_t T, bool F16>
-__global__ __launch_bounds__(256) void batch_closure_kernel(const uint32_t* __restrict__ d_all,
-                                                            E* __restrict__ W_all, uint32_t* __restrict__ p_all,
-                                                            uint32_t* __restrict__ rank_all,
-                                                            const ElectionMeta* __restrict__ meta,
-                                                            const uint32_t* __restrict__ order, uint32_t phase) {
+__device__ __forceinline__ void batch_closure_body(const uint32_t* __restrict__ d_all, E* __restrict__ W_all,
+                                                   uint32_t* __restrict__ p_all, uint32_t* __restrict__ rank_all,
+                                                   const ElectionMeta* __restrict__ meta,
+                                                   const uint32_t* __restrict__ order) {
     __shared__ __align__(16) BatchLds<E, T> L;
     const uint32_t t = threadIdx.x;
     const ElectionMeta em = meta[order[blockIdx.x]];
@@ -235,14 +245,10 @@ __global__ __launch_bounds__(256) void batch_closure_kernel(const uint32_t* __re
     const uint32_t* d = d_all + em.d_off;
     uint32_t* p = p_all + em.d_off;
     E* W = W_all + em.w_off;
-    if (C == 0) return;
-    const bool alone = em.piv == kNoRounds;
+    if (C == 0 || em.piv != kNoRounds) return;
     constexpr uint32_t bias = F16 ? kBatchF16Bias : 0u;
-    if (phase == 1 && alone) return;
-    if (phase == 1) goto tail;
 
-    batch_p0<E>(d, C, W, Cp, bias, L.u.S, L.u.S + kPT * kPP);
-    if (!alone) return;
+    batch_p0<E, 4>(d, C, W, Cp, bias, L.u.S, L.u.S + kPT * kPP);
 
     for (uint32_t r = 0, rounds = (C + T - 1) / T; r < rounds; ++r) {
         const uint32_t kb = r * T;
@@ -268,10 +274,51 @@ __global__ __launch_bounds__(256) void batch_closure_kernel(const uint32_t* __re
         }
     }
 
-tail:
     // output with zero diagonal, wins (:287-290) and the stable ranking (:292)
     __syncthreads();  // the closure's last LDS use (the union) ends here
-    batch_tail<E>(W, Cp, C, bias, p, rank_all + em.r_off, L.wins, L.u.S, L.u.S + kPT * kPP);
+    batch_tail<E, 4>(W, Cp, C, bias, p, rank_all + em.r_off, L.wins, L.u.S, L.u.S + kPT * kPP);
+}
+
+template <typename E, uint32_t T, bool F16>
+__global__ __launch_bounds__(256) void batch_closure_kernel(const uint32_t* __restrict__ d_all,
+                                                            E* __restrict__ W_all, uint32_t* __restrict__ p_all,
+                                                            uint32_t* __restrict__ rank_all,
+                                                            const ElectionMeta* __restrict__ meta,
+                                                            const uint32_t* __restrict__ order) {
+    batch_closure_body<E, T, F16>(d_all, W_all, p_all, rank_all, meta, order);
+}
+
+// The 32-bit form: its LDS (43.6 KB) admits three workgroups per CU; the bounds keep its
+// registers within three waves per SIMD.
+__global__ __launch_bounds__(256, 3) void batch_closure32_kernel(const uint32_t* __restrict__ d_all,
+                                                                 uint32_t* __restrict__ W_all,
+                                                                 uint32_t* __restrict__ p_all,
+                                                                 uint32_t* __restrict__ rank_all,
+                                                                 const ElectionMeta* __restrict__ meta,
+                                                                 const uint32_t* __restrict__ order) {
+    batch_closure_body<uint32_t, 32, false>(d_all, W_all, p_all, rank_all, meta, order);
+}
+
+// p0 (phase 0, before the first round) or the tail (phase 1, after the last pivot is
+// committed) of the round-closed elections `list`, one workgroup each: the passes of
+// batch_closure_kernel in a kernel needing 37 KB of LDS and few registers, so several
+// workgroups share a CU where the closure kernel fits two.
+template <bool F16>
+__global__ __launch_bounds__(256) void batch_edge_kernel(const uint32_t* __restrict__ d_all,
+                                                         uint16_t* __restrict__ W_all, uint32_t* __restrict__ p_all,
+                                                         uint32_t* __restrict__ rank_all,
+                                                         const ElectionMeta* __restrict__ meta,
+                                                         const uint32_t* __restrict__ list, uint32_t phase) {
+    __shared__ __align__(16) uint32_t AB[2 * kPT * kPP];
+    __shared__ uint32_t wins[kMaxBatchC];
+    const ElectionMeta em = meta[list[blockIdx.x]];
+    constexpr uint32_t bias = F16 ? kBatchF16Bias : 0u;
+    if (em.C == 0) return;
+    if (phase == 0)
+        batch_p0<uint16_t, 16>(d_all + em.d_off, em.C, W_all + em.w_off, em.Cp, bias, AB, AB + kPT * kPP);
+    else
+        batch_tail<uint16_t, 16>(W_all + em.w_off, em.Cp, em.C, bias, p_all + em.d_off, rank_all + em.r_off, wins, AB,
+                             AB + kPT * kPP);
 }
 
 // Round r of the round-closed elections, 16-bit: one workgroup per election for the
@@ -561,25 +608,27 @@ int batch_device_impl(DeviceContext* ctx, uint32_t E, const uint32_t* Cs, const 
             uint16_t* W16 = static_cast<uint16_t*>(wbuf);
             auto rounds16 = [&](auto f16_tag) {
                 constexpr bool F16 = decltype(f16_tag)::value;
-                batch_closure_kernel<uint16_t, 64, F16><<<E, kThreads, 0, s>>>(dbuf, W16, paths, ranking, dmeta, dorder, 0u);
-                if (large.empty()) return;
+                const uint32_t nl = (uint32_t)large.size();
+                if (nl)
+                    batch_edge_kernel<F16><<<nl, kThreads, 0, s>>>(dbuf, W16, paths, ranking, dmeta, dlarge, 0u);
+                batch_closure_kernel<uint16_t, 64, F16><<<E, kThreads, 0, s>>>(dbuf, W16, paths, ranking, dmeta, dorder);
+                if (!nl) return;
                 for (uint32_t r = 0; r < max_rounds; ++r) {
                     const uint32_t np = (uint32_t)(panel_off[r + 1] - panel_off[r]);
                     const uint32_t nt = (uint32_t)(tile_off[r + 1] - tile_off[r]);
                     batch_panel_kernel<64, F16><<<np, kThreads, 0, s>>>(W16, dmeta, dpanels + panel_off[r], r, pivots);
                     batch_tile_kernel<64, F16><<<nt, kThreads, 0, s>>>(W16, dmeta, dtiles + tile_off[r], r);
                 }
-                batch_commit_kernel<64><<<(uint32_t)large.size(), kThreads, 0, s>>>(W16, dmeta, dlarge, pivots);
-                batch_closure_kernel<uint16_t, 64, F16>
-                    <<<(uint32_t)large.size(), kThreads, 0, s>>>(dbuf, W16, paths, ranking, dmeta, dlarge, 1u);
+                batch_commit_kernel<64><<<nl, kThreads, 0, s>>>(W16, dmeta, dlarge, pivots);
+                batch_edge_kernel<F16><<<nl, kThreads, 0, s>>>(dbuf, W16, paths, ranking, dmeta, dlarge, 1u);
             };
             if (use16 && max_m <= kBatchF16MaxVoters)
                 rounds16(std::true_type{});
             else if (use16)
                 rounds16(std::false_type{});
             else
-                batch_closure_kernel<uint32_t, 32, false><<<E, kThreads, 0, s>>>(
-                    dbuf, static_cast<uint32_t*>(wbuf), paths, ranking, dmeta, dorder, 0u);
+                batch_closure32_kernel<<<E, kThreads, 0, s>>>(dbuf, static_cast<uint32_t*>(wbuf), paths, ranking,
+                                                              dmeta, dorder);
         }
         SE_CHECK_LAUNCH("batch closure launch failed");
         if (timing) (void)hipEventRecord(ev[2], s);
