@@ -68,6 +68,24 @@ __global__ __launch_bounds__(256) void reduce_slices_kernel(const uint32_t* __re
     }
 }
 
+// The same sum four columns at a time (n % 4 == 0; d 16-byte aligned with ldd % 4 == 0):
+// 16-byte loads keep more of each plane in flight (4096 x 4096, 5 planes: 125 us with one
+// column per load).
+__global__ __launch_bounds__(256) void reduce_slices4_kernel(const uint4* __restrict__ part, uint32_t slices,
+                                                             size_t plane4, uint32_t n4, uint4* __restrict__ d,
+                                                             size_t ldd4, uint32_t accumulate) {
+    const uint32_t i = blockIdx.x;
+    for (uint32_t j = threadIdx.x; j < n4; j += blockDim.x) {
+        uint4 s = accumulate ? d[(size_t)i * ldd4 + j] : make_uint4(0, 0, 0, 0);
+#pragma unroll 4
+        for (uint32_t k = 0; k < slices; ++k) {
+            const uint4 v = part[(size_t)k * plane4 + (size_t)i * n4 + j];
+            s = make_uint4(s.x + v.x, s.y + v.y, s.z + v.z, s.w + v.w);
+        }
+        d[(size_t)i * ldd4 + j] = s;
+    }
+}
+
 __global__ void zero_matrix_kernel(uint32_t* d, uint32_t n, size_t ldd) {
     const uint32_t i = blockIdx.x;
     for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) d[(size_t)i * ldd + j] = 0u;
@@ -170,7 +188,11 @@ int count_device_impl(DeviceContext* ctx, const void* pos, bool pos16, uint32_t 
                                     (uint32_t)slice_len, plane, acc, flag);
         SE_CHECK_LAUNCH("counter launch failed");
         if (sl > 1) {
-            reduce_slices_kernel<<<n, 256, 0, s>>>(part, sl, plane, n, d, ldd, p > 0 ? 1u : 0u);
+            if (n % 4 == 0 && ldd % 4 == 0 && ((uintptr_t)d & 15) == 0)
+                reduce_slices4_kernel<<<n, 256, 0, s>>>(reinterpret_cast<const uint4*>(part), sl, plane / 4, n / 4,
+                                                        reinterpret_cast<uint4*>(d), ldd / 4, p > 0 ? 1u : 0u);
+            else
+                reduce_slices_kernel<<<n, 256, 0, s>>>(part, sl, plane, n, d, ldd, p > 0 ? 1u : 0u);
             SE_CHECK_LAUNCH("counter launch failed");
         }
     }
