@@ -131,89 +131,94 @@ __device__ __forceinline__ void indep_tile(E* W, size_t ldw, uint32_t kb, uint32
 // w of the workgroup handles rows w, w + 4, ... of a tile, lane c column c.
 constexpr uint32_t kPT = 64, kPP = kPT + 1;
 
-// p0 (scaling_elections.py:103-112) of one election (d: C x C) into its padded work matrix
-// (Cp x Cp): v = d[i][j] + bias when d[i][j] > d[j][i], else 0 (diagonal and padding 0).
-// A, B: 2 x 64 x 65 u32 of LDS.
+// p0 (scaling_elections.py:103-112) of tile pair (I, J), I <= J, of one election (d: C x C)
+// into its padded work matrix (Cp x Cp): v = d[i][j] + bias when d[i][j] > d[j][i], else 0
+// (diagonal and padding 0).  A, B: 2 x 64 x 65 u32 of LDS.  Starts with a barrier.
 // U: rows staged per unrolled step (loads in flight per thread: 2 U).
+template <typename E, uint32_t U>
+__device__ __forceinline__ void p0_pair(const uint32_t* __restrict__ d, uint32_t C, E* __restrict__ W, uint32_t Cp,
+                                        uint32_t bias, uint32_t I, uint32_t J, uint32_t* A, uint32_t* B) {
+    const uint32_t t = threadIdx.x, c = t & (kPT - 1), r0 = t / kPT;
+    __syncthreads();
+    // unconditional loads at clamped indices (cells outside C x C count as 0), so the loads
+    // of U rows issue together instead of one round trip each
+#pragma unroll U
+    for (uint32_t r = r0; r < kPT; r += kThreads / kPT) {
+        const uint32_t i = I * kPT + r, j = J * kPT + c, i2 = J * kPT + r, j2 = I * kPT + c;
+        const uint32_t x = d[(size_t)min(i, C - 1) * C + min(j, C - 1)];
+        const uint32_t y = d[(size_t)min(i2, C - 1) * C + min(j2, C - 1)];
+        A[r * kPP + c] = (i < C && j < C) ? x : 0u;
+        B[r * kPP + c] = (i2 < C && j2 < C) ? y : 0u;
+    }
+    __syncthreads();
+    for (uint32_t r = r0; r < kPT; r += kThreads / kPT) {
+        const uint32_t i = I * kPT + r, j = J * kPT + c;
+        if (i < Cp && j < Cp) {
+            const uint32_t x = A[r * kPP + c], y = B[c * kPP + r];
+            W[(size_t)i * Cp + j] = (E)((i != j && x > y) ? x + bias : 0u);
+        }
+        const uint32_t i2 = J * kPT + r, j2 = I * kPT + c;
+        if (I != J && i2 < Cp && j2 < Cp) {
+            const uint32_t x = B[r * kPP + c], y = A[c * kPP + r];
+            W[(size_t)i2 * Cp + j2] = (E)(x > y ? x + bias : 0u);
+        }
+    }
+}
+
+// p0 of a whole election, pair by pair.
 template <typename E, uint32_t U>
 __device__ void batch_p0(const uint32_t* __restrict__ d, uint32_t C, E* __restrict__ W, uint32_t Cp,
                          uint32_t bias, uint32_t* A, uint32_t* B) {
-    const uint32_t t = threadIdx.x, c = t & (kPT - 1), r0 = t / kPT;
     const uint32_t nt = (Cp + kPT - 1) / kPT;
     for (uint32_t I = 0; I < nt; ++I)
-        for (uint32_t J = I; J < nt; ++J) {
-            __syncthreads();
-            // unconditional loads at clamped indices (cells outside C x C count as 0), so
-            // the loads of U rows issue together instead of one round trip each
-#pragma unroll U
-            for (uint32_t r = r0; r < kPT; r += kThreads / kPT) {
-                const uint32_t i = I * kPT + r, j = J * kPT + c, i2 = J * kPT + r, j2 = I * kPT + c;
-                const uint32_t x = d[(size_t)min(i, C - 1) * C + min(j, C - 1)];
-                const uint32_t y = d[(size_t)min(i2, C - 1) * C + min(j2, C - 1)];
-                A[r * kPP + c] = (i < C && j < C) ? x : 0u;
-                B[r * kPP + c] = (i2 < C && j2 < C) ? y : 0u;
-            }
-            __syncthreads();
-            for (uint32_t r = r0; r < kPT; r += kThreads / kPT) {
-                const uint32_t i = I * kPT + r, j = J * kPT + c;
-                if (i < Cp && j < Cp) {
-                    const uint32_t x = A[r * kPP + c], y = B[c * kPP + r];
-                    W[(size_t)i * Cp + j] = (E)((i != j && x > y) ? x + bias : 0u);
-                }
-                const uint32_t i2 = J * kPT + r, j2 = I * kPT + c;
-                if (I != J && i2 < Cp && j2 < Cp) {
-                    const uint32_t x = B[r * kPP + c], y = A[c * kPP + r];
-                    W[(size_t)i2 * Cp + j2] = (E)(x > y ? x + bias : 0u);
-                }
-            }
-        }
+        for (uint32_t J = I; J < nt; ++J) p0_pair<E, U>(d, C, W, Cp, bias, I, J, A, B);
     __syncthreads();
 }
 
-// Output p (C x C, zero diagonal, bias removed), wins[i] = #{j != i : W[i][j] > W[j][i]}
-// (scaling_elections.py:287-290; the offset keeps the order) and the stable ranking by
-// descending wins, ties in ascending index (:292): rank(i) = #{j : w_j > w_i or (w_j = w_i
-// and j < i)}.  wins: C words of LDS; A, B as in batch_p0.
+// Tail of tile pair (I, J), I <= J: output p (C x C, zero diagonal, bias removed) of both
+// tiles, and wins[i] += #{j in the pair's columns, j != i : W[i][j] > W[j][i]}
+// (scaling_elections.py:287-290; the offset keeps the order) for the rows of both tiles,
+// added atomically (wins: LDS or global).  A, B as in p0_pair.  Starts with a barrier.
 template <typename E, uint32_t U>
-__device__ void batch_tail(const E* __restrict__ W, uint32_t Cp, uint32_t C, uint32_t bias, uint32_t* __restrict__ p,
-                           uint32_t* __restrict__ rank_out, uint32_t* wins, uint32_t* A, uint32_t* B) {
+__device__ __forceinline__ void tail_pair(const E* __restrict__ W, uint32_t Cp, uint32_t C, uint32_t bias,
+                                          uint32_t* __restrict__ p, uint32_t I, uint32_t J, uint32_t* wins,
+                                          uint32_t* A, uint32_t* B) {
     const uint32_t t = threadIdx.x, c = t & (kPT - 1), r0 = t / kPT;
-    for (uint32_t i = t; i < C; i += kThreads) wins[i] = 0;
-    const uint32_t nt = (C + kPT - 1) / kPT;
-    for (uint32_t I = 0; I < nt; ++I)
-        for (uint32_t J = I; J < nt; ++J) {
-            __syncthreads();
-            // unconditional loads at clamped indices (cells outside C x C count as 0), so the
-            // loads of U rows issue together
-#pragma unroll U
-            for (uint32_t r = r0; r < kPT; r += kThreads / kPT) {
-                const uint32_t i = I * kPT + r, j = J * kPT + c, i2 = J * kPT + r, j2 = I * kPT + c;
-                const uint32_t x = W[(size_t)min(i, C - 1) * Cp + min(j, C - 1)];
-                const uint32_t y = W[(size_t)min(i2, C - 1) * Cp + min(j2, C - 1)];
-                A[r * kPP + c] = (i < C && j < C) ? x : 0u;
-                B[r * kPP + c] = (i2 < C && j2 < C) ? y : 0u;
-            }
-            __syncthreads();
-            uint32_t beats = 0;  // column j = J * 64 + c over this tile's rows (I != J)
-            for (uint32_t r = r0; r < kPT; r += kThreads / kPT) {
-                const uint32_t i = I * kPT + r, j = J * kPT + c;
-                const bool in = i < C && j < C;
-                const uint32_t x = A[r * kPP + c], y = B[c * kPP + r];
-                if (in) p[(size_t)i * C + j] = (i == j || x == 0) ? 0u : x - bias;
-                // row i is this wave's alone: count its wins with one ballot
-                const unsigned long long won = __ballot(in && i != j && x > y);
-                if (c == 0 && i < C) wins[i] += (uint32_t)__popcll(won);
-                if (I != J) {
-                    beats += (in && y > x) ? 1u : 0u;
-                    const uint32_t i2 = J * kPT + r, j2 = I * kPT + c;
-                    const uint32_t x2 = B[r * kPP + c];
-                    if (i2 < C && j2 < C) p[(size_t)i2 * C + j2] = x2 == 0 ? 0u : x2 - bias;
-                }
-            }
-            if (beats) atomicAdd(&wins[J * kPT + c], beats);
-        }
     __syncthreads();
-    for (uint32_t i = t; i < C; i += kThreads) {
+    // unconditional loads at clamped indices (cells outside C x C count as 0), so the loads
+    // of U rows issue together
+#pragma unroll U
+    for (uint32_t r = r0; r < kPT; r += kThreads / kPT) {
+        const uint32_t i = I * kPT + r, j = J * kPT + c, i2 = J * kPT + r, j2 = I * kPT + c;
+        const uint32_t x = W[(size_t)min(i, C - 1) * Cp + min(j, C - 1)];
+        const uint32_t y = W[(size_t)min(i2, C - 1) * Cp + min(j2, C - 1)];
+        A[r * kPP + c] = (i < C && j < C) ? x : 0u;
+        B[r * kPP + c] = (i2 < C && j2 < C) ? y : 0u;
+    }
+    __syncthreads();
+    uint32_t beats = 0;  // column j = J * 64 + c over this tile's rows (I != J)
+    for (uint32_t r = r0; r < kPT; r += kThreads / kPT) {
+        const uint32_t i = I * kPT + r, j = J * kPT + c;
+        const bool in = i < C && j < C;
+        const uint32_t x = A[r * kPP + c], y = B[c * kPP + r];
+        if (in) p[(size_t)i * C + j] = (i == j || x == 0) ? 0u : x - bias;
+        // row i is this wave's alone within the pair: count its wins with one ballot
+        const unsigned long long won = __ballot(in && i != j && x > y);
+        if (c == 0 && won) atomicAdd(&wins[i], (uint32_t)__popcll(won));
+        if (I != J) {
+            beats += (in && y > x) ? 1u : 0u;
+            const uint32_t i2 = J * kPT + r, j2 = I * kPT + c;
+            const uint32_t x2 = B[r * kPP + c];
+            if (i2 < C && j2 < C) p[(size_t)i2 * C + j2] = x2 == 0 ? 0u : x2 - bias;
+        }
+    }
+    if (beats) atomicAdd(&wins[J * kPT + c], beats);
+}
+
+// The stable ranking by descending wins, ties in ascending index (:292):
+// rank(i) = #{j : w_j > w_i or (w_j = w_i and j < i)}.  wins: C words of LDS.
+__device__ __forceinline__ void rank_from_wins(const uint32_t* wins, uint32_t C, uint32_t* __restrict__ rank_out) {
+    for (uint32_t i = threadIdx.x; i < C; i += kThreads) {
         const uint32_t wi = wins[i];
         uint32_t rank = 0;
         for (uint32_t j = 0; j < C; ++j) {
@@ -224,6 +229,19 @@ __device__ void batch_tail(const E* __restrict__ W, uint32_t Cp, uint32_t C, uin
     }
 }
 
+// Output, wins and ranking of a whole election, pair by pair.  wins: C words of LDS; A, B
+// as in p0_pair.
+template <typename E, uint32_t U>
+__device__ void batch_tail(const E* __restrict__ W, uint32_t Cp, uint32_t C, uint32_t bias, uint32_t* __restrict__ p,
+                           uint32_t* __restrict__ rank_out, uint32_t* wins, uint32_t* A, uint32_t* B) {
+    for (uint32_t i = threadIdx.x; i < C; i += kThreads) wins[i] = 0;
+    const uint32_t nt = (C + kPT - 1) / kPT;
+    for (uint32_t I = 0; I < nt; ++I)
+        for (uint32_t J = I; J < nt; ++J) tail_pair<E, U>(W, Cp, C, bias, p, I, J, wins, A, B);
+    __syncthreads();
+    rank_from_wins(wins, C, rank_out);
+}
+
 // Offset of the f16-ordered batch closure: nonzero p0 values v <= m <= 30720 are stored as
 // v + 0x3FF, positive f16 normals in [0x0400, 0x7BFF] whose unsigned order is their value
 // order; the pivot and panel code compares them as integers, the tiles as f16.
@@ -232,7 +250,8 @@ constexpr uint32_t kBatchF16MaxVoters = 0x7C00u - 0x0400u;
 
 // The elections closed alone (piv == kNoRounds), one workgroup each: p0 into the work
 // matrix, the closure, then the tail (output, wins, ranking).  The round-closed elections
-// exit at once (batch_edge_kernel runs their p0 and tail).
+// exit at once (batch_p0_pairs_kernel, batch_tail_pairs_kernel and batch_rank_kernel
+// handle their p0 and tail).
 template <typename E, uint32_t T, bool F16>
 __device__ __forceinline__ void batch_closure_body(const uint32_t* __restrict__ d_all, E* __restrict__ W_all,
                                                    uint32_t* __restrict__ p_all, uint32_t* __restrict__ rank_all,
@@ -299,26 +318,50 @@ __global__ __launch_bounds__(256, 3) void batch_closure32_kernel(const uint32_t*
     batch_closure_body<uint32_t, 32, false>(d_all, W_all, p_all, rank_all, meta, order);
 }
 
-// p0 (phase 0, before the first round) or the tail (phase 1, after the last pivot is
-// committed) of the round-closed elections `list`, one workgroup each: the passes of
-// batch_closure_kernel in a kernel needing 37 KB of LDS and few registers, so several
-// workgroups share a CU where the closure kernel fits two.
+// p0 of the round-closed elections, one workgroup per tile pair (items: election, I, J,
+// over the padded side).  The (I, I) workgroups also zero the election's win counts of rows
+// I, which batch_tail_pairs_kernel adds to after the rounds.
 template <bool F16>
-__global__ __launch_bounds__(256) void batch_edge_kernel(const uint32_t* __restrict__ d_all,
-                                                         uint16_t* __restrict__ W_all, uint32_t* __restrict__ p_all,
-                                                         uint32_t* __restrict__ rank_all,
-                                                         const ElectionMeta* __restrict__ meta,
-                                                         const uint32_t* __restrict__ list, uint32_t phase) {
+__global__ __launch_bounds__(256) void batch_p0_pairs_kernel(const uint32_t* __restrict__ d_all,
+                                                             uint16_t* __restrict__ W_all,
+                                                             const ElectionMeta* __restrict__ meta,
+                                                             const uint4* __restrict__ items,
+                                                             uint32_t* __restrict__ wins_all) {
     __shared__ __align__(16) uint32_t AB[2 * kPT * kPP];
+    const uint4 it = items[blockIdx.x];
+    const ElectionMeta em = meta[it.x];
+    constexpr uint32_t bias = F16 ? kBatchF16Bias : 0u;
+    p0_pair<uint16_t, 16>(d_all + em.d_off, em.C, W_all + em.w_off, em.Cp, bias, it.y, it.z, AB, AB + kPT * kPP);
+    if (it.y == it.z && threadIdx.x < kPT && it.y * kPT + threadIdx.x < em.C)
+        wins_all[em.r_off + it.y * kPT + threadIdx.x] = 0u;
+}
+
+// Output and win counts of the round-closed elections, one workgroup per tile pair (items
+// over the C x C domain), after the last pivot is committed.
+template <bool F16>
+__global__ __launch_bounds__(256) void batch_tail_pairs_kernel(const uint16_t* __restrict__ W_all,
+                                                               uint32_t* __restrict__ p_all,
+                                                               const ElectionMeta* __restrict__ meta,
+                                                               const uint4* __restrict__ items,
+                                                               uint32_t* __restrict__ wins_all) {
+    __shared__ __align__(16) uint32_t AB[2 * kPT * kPP];
+    const uint4 it = items[blockIdx.x];
+    const ElectionMeta em = meta[it.x];
+    constexpr uint32_t bias = F16 ? kBatchF16Bias : 0u;
+    tail_pair<uint16_t, 16>(W_all + em.w_off, em.Cp, em.C, bias, p_all + em.d_off, it.y, it.z,
+                            wins_all + em.r_off, AB, AB + kPT * kPP);
+}
+
+// Rankings of the round-closed elections from their win counts, one workgroup each.
+__global__ __launch_bounds__(256) void batch_rank_kernel(const ElectionMeta* __restrict__ meta,
+                                                         const uint32_t* __restrict__ list,
+                                                         const uint32_t* __restrict__ wins_all,
+                                                         uint32_t* __restrict__ rank_all) {
     __shared__ uint32_t wins[kMaxBatchC];
     const ElectionMeta em = meta[list[blockIdx.x]];
-    constexpr uint32_t bias = F16 ? kBatchF16Bias : 0u;
-    if (em.C == 0) return;
-    if (phase == 0)
-        batch_p0<uint16_t, 16>(d_all + em.d_off, em.C, W_all + em.w_off, em.Cp, bias, AB, AB + kPT * kPP);
-    else
-        batch_tail<uint16_t, 16>(W_all + em.w_off, em.Cp, em.C, bias, p_all + em.d_off, rank_all + em.r_off, wins, AB,
-                             AB + kPT * kPP);
+    for (uint32_t i = threadIdx.x; i < em.C; i += kThreads) wins[i] = wins_all[em.r_off + i];
+    __syncthreads();
+    rank_from_wins(wins, em.C, rank_all + em.r_off);
 }
 
 // Round r of the round-closed elections, 16-bit: one workgroup per election for the
@@ -578,21 +621,40 @@ int batch_device_impl(DeviceContext* ctx, uint32_t E, const uint32_t* Cs, const 
         }
         panel_off[max_rounds] = panel_items.size();
         tile_off[max_rounds] = tile_items.size();
+        // p0 and tail tile pairs (I <= J) of the round-closed elections: p0 over the padded
+        // side, the tail over C x C
+        std::vector<uint4> p0_pairs, tail_pairs;
+        for (uint32_t e : large) {
+            const uint32_t np = meta[e].Cp / kPT, nc = (Cs[e] + kPT - 1) / kPT;
+            for (uint32_t I = 0; I < np; ++I)
+                for (uint32_t J = I; J < np; ++J) {
+                    p0_pairs.push_back(make_uint4(e, I, J, 0));
+                    if (J < nc) tail_pairs.push_back(make_uint4(e, I, J, 0));
+                }
+        }
 
         uint32_t* pivots = nullptr;
         uint2* dpanels = nullptr;
         uint4* dtiles = nullptr;
         uint32_t* dlarge = nullptr;
+        uint4 *dp0_pairs = nullptr, *dtail_pairs = nullptr;
+        uint32_t* dwins = nullptr;
         if (!large.empty()) {
             const size_t tb = round_up(tile_items.size() * sizeof(uint4), 256);
             const size_t pb = round_up(panel_items.size() * sizeof(uint2), 256);
             const size_t lb = round_up(large.size() * 4, 256);
+            const size_t qb = round_up(p0_pairs.size() * sizeof(uint4), 256);
+            const size_t ub = round_up(tail_pairs.size() * sizeof(uint4), 256);
+            const size_t wb = round_up(r_total * 4, 256);
             pivots = static_cast<uint32_t*>(scratch(ctx, kSlotBatchPivot, large.size() * 2 * T * T * 4));
-            char* rbuf = static_cast<char*>(scratch(ctx, kSlotBatchRounds, tb + pb + lb));
+            char* rbuf = static_cast<char*>(scratch(ctx, kSlotBatchRounds, tb + pb + lb + qb + ub + wb));
             if (!pivots || !rbuf) return SE_ERR_ALLOC;
             dtiles = reinterpret_cast<uint4*>(rbuf);
             dpanels = reinterpret_cast<uint2*>(rbuf + tb);
             dlarge = reinterpret_cast<uint32_t*>(rbuf + tb + pb);
+            dp0_pairs = reinterpret_cast<uint4*>(rbuf + tb + pb + lb);
+            dtail_pairs = reinterpret_cast<uint4*>(rbuf + tb + pb + lb + qb);
+            dwins = reinterpret_cast<uint32_t*>(rbuf + tb + pb + lb + qb + ub);
             if (!hip_ok(hipMemcpyAsync(dtiles, tile_items.data(), tile_items.size() * sizeof(uint4),
                                        hipMemcpyHostToDevice, s),
                         "Failed to copy data from host to device") ||
@@ -600,6 +662,12 @@ int batch_device_impl(DeviceContext* ctx, uint32_t E, const uint32_t* Cs, const 
                                        hipMemcpyHostToDevice, s),
                         "Failed to copy data from host to device") ||
                 !hip_ok(hipMemcpyAsync(dlarge, large.data(), large.size() * 4, hipMemcpyHostToDevice, s),
+                        "Failed to copy data from host to device") ||
+                !hip_ok(hipMemcpyAsync(dp0_pairs, p0_pairs.data(), p0_pairs.size() * sizeof(uint4),
+                                       hipMemcpyHostToDevice, s),
+                        "Failed to copy data from host to device") ||
+                !hip_ok(hipMemcpyAsync(dtail_pairs, tail_pairs.data(), tail_pairs.size() * sizeof(uint4),
+                                       hipMemcpyHostToDevice, s),
                         "Failed to copy data from host to device"))
                 return SE_ERR_COPY;
         }
@@ -610,7 +678,8 @@ int batch_device_impl(DeviceContext* ctx, uint32_t E, const uint32_t* Cs, const 
                 constexpr bool F16 = decltype(f16_tag)::value;
                 const uint32_t nl = (uint32_t)large.size();
                 if (nl)
-                    batch_edge_kernel<F16><<<nl, kThreads, 0, s>>>(dbuf, W16, paths, ranking, dmeta, dlarge, 0u);
+                    batch_p0_pairs_kernel<F16><<<(uint32_t)p0_pairs.size(), kThreads, 0, s>>>(dbuf, W16, dmeta,
+                                                                                             dp0_pairs, dwins);
                 batch_closure_kernel<uint16_t, 64, F16><<<E, kThreads, 0, s>>>(dbuf, W16, paths, ranking, dmeta, dorder);
                 if (!nl) return;
                 for (uint32_t r = 0; r < max_rounds; ++r) {
@@ -620,7 +689,9 @@ int batch_device_impl(DeviceContext* ctx, uint32_t E, const uint32_t* Cs, const 
                     batch_tile_kernel<64, F16><<<nt, kThreads, 0, s>>>(W16, dmeta, dtiles + tile_off[r], r);
                 }
                 batch_commit_kernel<64><<<nl, kThreads, 0, s>>>(W16, dmeta, dlarge, pivots);
-                batch_edge_kernel<F16><<<nl, kThreads, 0, s>>>(dbuf, W16, paths, ranking, dmeta, dlarge, 1u);
+                batch_tail_pairs_kernel<F16>
+                    <<<(uint32_t)tail_pairs.size(), kThreads, 0, s>>>(W16, paths, dmeta, dtail_pairs, dwins);
+                batch_rank_kernel<<<nl, kThreads, 0, s>>>(dmeta, dlarge, dwins, ranking);
             };
             if (use16 && max_m <= kBatchF16MaxVoters)
                 rounds16(std::true_type{});
