@@ -448,11 +448,6 @@ __global__ void init_range_kernel(uint32_t* p) {
     }
 }
 
-// Zeroes a few words with a kernel (keeps the pipeline free of memset nodes).
-__global__ void zero_words_kernel(uint32_t* p, uint32_t count) {
-    for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) p[i] = 0u;
-}
-
 // ---------------------------------------------------------------------------------------
 // Host driver.
 
