@@ -63,6 +63,8 @@ __global__ __launch_bounds__(256) void reduce_slices_kernel(const uint32_t* __re
     const uint32_t i = blockIdx.x;
     for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) {
         uint32_t s = accumulate ? d[(size_t)i * ldd + j] : 0u;
+        // small elections sum many planes (one per 32+ voters): keep 8 loads in flight
+#pragma unroll 8
         for (uint32_t k = 0; k < slices; ++k) s += part[(size_t)k * plane + (size_t)i * n + j];
         d[(size_t)i * ldd + j] = s;
     }
