@@ -79,7 +79,7 @@ __global__ __launch_bounds__(256) void reduce_slices4_kernel(const uint4* __rest
     const uint32_t i = blockIdx.x;
     for (uint32_t j = threadIdx.x; j < n4; j += blockDim.x) {
         uint4 s = accumulate ? d[(size_t)i * ldd4 + j] : make_uint4(0, 0, 0, 0);
-#pragma unroll 4
+#pragma unroll 8
         for (uint32_t k = 0; k < slices; ++k) {
             const uint4 v = part[(size_t)k * plane4 + (size_t)i * n4 + j];
             s = make_uint4(s.x + v.x, s.y + v.y, s.z + v.z, s.w + v.w);
