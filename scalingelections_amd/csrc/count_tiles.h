@@ -214,9 +214,9 @@ __device__ __forceinline__ void stage_words(const Pos* __restrict__ pos, size_t 
             s = __builtin_amdgcn_udot2(__builtin_bit_cast(ushort2_t, word), ushort2_t{1, 1}, s, false);
         return word;
     };
-    // words per step: 4, or 2 for bytes in split staging (16 loads in flight did not fit
-    // beside the split kernels' accumulators and were serialized through one register)
-    constexpr uint32_t Q = (SPLIT && K == 4) ? 2u : 4u;
+    // words per step (K loads each): on rounds inside the slice all of a step's loads are
+    // in flight together; the last round's clamped loads may issue one at a time
+    constexpr uint32_t Q = 4u;
     auto run = [&](auto whole_tag) {
         uint32_t w = first;
         for (; w + (Q - 1) * step < kVW; w += Q * step) {
