@@ -305,6 +305,19 @@ def test_ranking_matches_oracle(gpu_lib, oracle_mod, n):
     assert get_winner_and_ranking(cands, p) == oracle_mod.winner_and_ranking(cands, p)
 
 
+@pytest.mark.parametrize("n", [4000, 4160])
+def test_wins_with_multi_block_column_segments(gpu_lib, oracle_mod, n):
+    """Sizes where each wins workgroup covers several column blocks, the last one ragged
+    (4000) or whole (4160): counts and ranking equal the oracle's on a matrix full of ties."""
+    from scalingelections_amd import wins_and_order
+
+    print(f"wins segments n={n}", flush=True)
+    p = np.random.default_rng(n).integers(0, 4, (n, n)).astype(np.uint32)
+    wins, order = wins_and_order(p)
+    assert wins.tolist() == oracle_mod.wins(p).tolist()
+    assert order.tolist() == oracle_mod.ranking(p).tolist()
+
+
 def test_ranking_empty_candidates_raises_like_reference(gpu_lib):
     from scalingelections_amd import get_winner_and_ranking
 
