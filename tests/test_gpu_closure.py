@@ -246,19 +246,42 @@ def _count_np(pos: np.ndarray) -> np.ndarray:
 
 
 @pytest.mark.parametrize("budget_planes", [1, 2])
-def test_counter_passes_beyond_16bit_slices(gpu_lib, monkeypatch, budget_planes):
+@pytest.mark.parametrize("n", [130, 132])
+def test_counter_passes_beyond_16bit_slices(gpu_lib, monkeypatch, budget_planes, n):
     """More voters than one 16-bit-packed slice holds (65534): the counter runs several
     passes that add into d.  A partial-plane budget of 1 or 2 planes forces passes of one
-    slice (direct accumulation) and of two slices (partial planes + accumulating reduce)."""
+    slice (direct accumulation) and of two slices (partial planes + accumulating reduce,
+    one column per load for n = 130, four for n = 132)."""
     from scalingelections_amd import count_positions
 
-    n, m = 130, 140000
+    m = 140000
     rng = np.random.default_rng(11)
     pos = np.argsort(rng.random((m, n)), axis=1).astype(np.int32)
     monkeypatch.setenv("SE_COUNT_PARTIAL_BYTES", str(budget_planes * n * n * 4))
     d = count_positions(pos)
     assert np.array_equal(d, _count_np(pos))
     assert np.array_equal(d.astype(np.int64) + d.T, m * (1 - np.eye(n, dtype=np.int64)))
+
+
+@pytest.mark.parametrize("stride", [0, 4, 3])
+def test_count_device_caller_stride(gpu_lib, stride):
+    """se_count_device into a caller buffer whose row stride is n + stride: the slice planes
+    are summed four columns per load when the stride keeps rows 16-byte aligned (0, 4) and
+    one column per load otherwise (3); cells past n in each row are left untouched."""
+    from scalingelections_amd import _lib, devmem, synth
+
+    n, m = 256, 3000
+    print(f"count device stride n+{stride}", flush=True)
+    pos = synth.positions_from_rankings(synth.impartial_culture(m, n, 4))
+    ld = n + stride
+    pd = devmem.DeviceBuffer.from_array(np.ascontiguousarray(pos, np.int32))
+    fill = np.full((n, ld), 0xDEADBEEF, np.uint32)
+    dd = devmem.DeviceBuffer.from_array(fill)
+    _lib.check(_lib.lib().se_count_device(pd.ptr, m, n, dd.ptr, ld, None, None))
+    devmem.device_synchronize()
+    got = dd.download(np.empty((n, ld), np.uint32))
+    assert np.array_equal(got[:, :n], _count_np(pos))
+    assert (got[:, n:] == 0xDEADBEEF).all()
 
 
 @pytest.mark.parametrize("n,m", [(130, 140000), (100, 70000), (300, 66000), (300, 65535)])
