@@ -699,7 +699,11 @@ int closure_device_ex(DeviceContext* ctx, const void* d_any, bool d16, uint32_t 
     } else {
         uint32_t* dmax = static_cast<uint32_t*>(scratch(ctx, kSlotSmall, kRangeSlots * kRangePitch * 4));
         if (!dmax) return SE_ERR_ALLOC;
-        uint32_t slots[kRangeSlots * kRangePitch];
+        // read back through pinned memory: a pageable destination goes through the
+        // runtime's own staging and takes tens of microseconds longer
+        static_assert(kRangeSlots * kRangePitch * sizeof(uint32_t) <= kSmallPinnedBytes, "range slots");
+        uint32_t* slots = small_pinned(ctx);
+        if (!slots) return SE_ERR_ALLOC;
         tm.mark(kKindInit);
         init_range_kernel<<<1, 64, 0, s>>>(dmax);
         const uint32_t ntn = (n + kInitTile - 1) / kInitTile;
@@ -709,7 +713,7 @@ int closure_device_ex(DeviceContext* ctx, const void* d_any, bool d16, uint32_t 
             p0_max_kernel<uint32_t><<<init_grid(ntn), kThreads, 0, s>>>(d32, n, ld, ntn, dmax);
         tm.mark(kKindInit);
         SE_CHECK_LAUNCH("p0 init launch failed");
-        if (!hip_ok(hipMemcpyAsync(slots, dmax, sizeof(slots), hipMemcpyDeviceToHost, s),
+        if (!hip_ok(hipMemcpyAsync(slots, dmax, kRangeSlots * kRangePitch * sizeof(uint32_t), hipMemcpyDeviceToHost, s),
                     "Failed to copy data from device to host") ||
             !hip_ok(hipStreamSynchronize(s), "CUDA operations did not complete successfully"))
             return SE_ERR_COPY;

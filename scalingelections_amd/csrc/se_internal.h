@@ -48,6 +48,7 @@ struct DeviceContext {
     size_t cap[kSlotCount] = {};
     void* pinned[2] = {nullptr, nullptr};          // pinned staging chunks (transfer.hip)
     hipEvent_t pinned_ev[2] = {nullptr, nullptr};  // last DMA of each chunk
+    uint32_t* host_small = nullptr;                // pinned 4 KB for small read-backs (transfer.hip)
 };
 
 // Host <-> device matrix copies through the context's pinned staging chunks
@@ -58,6 +59,10 @@ int upload_rows(DeviceContext* ctx, hipStream_t s, void* dev, const void* host, 
 int download_rows(DeviceContext* ctx, hipStream_t s, void* host, size_t host_pitch, const void* dev, size_t rows,
                   size_t row_bytes);
 void free_pinned(DeviceContext* ctx);
+// The context's pinned 4 KB read-back buffer (allocated on first use), or nullptr with the
+// error set.
+uint32_t* small_pinned(DeviceContext* ctx);
+constexpr size_t kSmallPinnedBytes = 4096;
 // 16-bit transit: u32 host matrix narrowed into a u16 device matrix (SE_NARROW_FAILED when
 // a value exceeds 65535), and a u16 device matrix widened into a u32 host matrix.
 constexpr int SE_NARROW_FAILED = 1;

@@ -300,7 +300,19 @@ int download_rows_widen16(DeviceContext* ctx, hipStream_t s, uint32_t* host, siz
     return SE_OK;
 }
 
+uint32_t* small_pinned(DeviceContext* ctx) {
+    if (!ctx->host_small &&
+        !hip_ok(hipHostMalloc(reinterpret_cast<void**>(&ctx->host_small), kSmallPinnedBytes, hipHostMallocDefault),
+                "Failed to allocate pinned host memory")) {
+        ctx->host_small = nullptr;
+        return nullptr;
+    }
+    return ctx->host_small;
+}
+
 void free_pinned(DeviceContext* ctx) {
+    if (ctx->host_small) (void)hipHostFree(ctx->host_small);
+    ctx->host_small = nullptr;
     for (int b = 0; b < 2; ++b) {
         if (ctx->pinned[b]) (void)hipHostFree(ctx->pinned[b]);
         if (ctx->pinned_ev[b]) (void)hipEventDestroy(ctx->pinned_ev[b]);
