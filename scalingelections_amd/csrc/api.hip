@@ -326,15 +326,16 @@ int se_count(const int32_t* positions, uint32_t m, uint32_t n, uint32_t* prefs, 
         (void)hipStreamSynchronize(s);
         return rc;
     }
-    uint32_t bad = 0;
-    if (!hip_ok(hipMemcpyAsync(&bad, flag, sizeof(bad), hipMemcpyDeviceToHost, s),
+    uint32_t* bad = small_pinned(ctx);  // pinned: a pageable 4-byte read-back is staged by the runtime
+    if (!bad) return SE_ERR_ALLOC;
+    if (!hip_ok(hipMemcpyAsync(bad, flag, sizeof(uint32_t), hipMemcpyDeviceToHost, s),
                 "Failed to copy data from device to host"))
         return SE_ERR_COPY;
     rc = d16 ? download_rows_widen16(ctx, s, prefs, prefs_stride * sizeof(uint32_t), dn, n, n)
              : download_rows(ctx, s, prefs, prefs_stride * sizeof(uint32_t), d, n, n * sizeof(uint32_t));
     if (rc != SE_OK) return rc;
     if (!hip_ok(hipStreamSynchronize(s), "CUDA transfers did not complete successfully")) return SE_ERR_COPY;
-    if (bad) {
+    if (*bad) {
         set_error("positions out of range: every ballot row must be a permutation of 0..n-1");
         return SE_ERR_INVALID;
     }
@@ -498,12 +499,13 @@ int se_positions16_device(const int32_t* src_dev, size_t count, uint16_t* dst_de
     int rc = clear_flag(flag, s);
     if (rc == SE_OK) rc = narrow_positions16(src_dev, count, dst_dev, flag, s);
     if (rc != SE_OK) return rc;
-    uint32_t bad = 0;
-    if (!hip_ok(hipMemcpyAsync(&bad, flag, sizeof(bad), hipMemcpyDeviceToHost, s),
+    uint32_t* bad = small_pinned(ctx);
+    if (!bad) return SE_ERR_ALLOC;
+    if (!hip_ok(hipMemcpyAsync(bad, flag, sizeof(uint32_t), hipMemcpyDeviceToHost, s),
                 "Failed to copy data from device to host") ||
         !hip_ok(hipStreamSynchronize(s), "CUDA operations did not complete successfully"))
         return SE_ERR_COPY;
-    if (bad) {
+    if (*bad) {
         set_error("positions out of range: a value lies outside 0..65535");
         return SE_ERR_INVALID;
     }
