@@ -426,6 +426,77 @@ __global__ __launch_bounds__(256) void small_closure_kernel(const D* __restrict_
         }
 }
 
+// The same for a 16-bit input (every value <= 65535, the host entry's 16-bit transit):
+// each lane keeps its 8 x 8 cells as 32 packed pairs (columns tx + 32 h and tx + 32 h + 16)
+// and updates two cells per v_pk_min_u16 / v_pk_max_u16, half the VALU work per pivot.  The
+// pivot column is published with its value in both halves, the pivot row as the pairs the
+// lanes hold, so every lane reads one word per register.
+template <typename O>
+__global__ __launch_bounds__(256) void small_closure16_kernel(const uint16_t* __restrict__ d, uint32_t n, size_t ld,
+                                                              O* __restrict__ out, size_t ldo) {
+    __shared__ uint32_t rowbuf[2][kSmallN / 2], colbuf[2][kSmallN];
+    const uint32_t t = threadIdx.x, ty = t >> 4, tx = t & 15;
+    u16x2 cell[8][4];
+#pragma unroll
+    for (uint32_t r = 0; r < 8; ++r)
+#pragma unroll
+        for (uint32_t h = 0; h < 4; ++h) {
+            const uint32_t i = ty + 16 * r;
+            uint32_t v[2];
+#pragma unroll
+            for (uint32_t e = 0; e < 2; ++e) {
+                const uint32_t j = tx + 32 * h + 16 * e;
+                v[e] = 0;
+                if (i < n && j < n && i != j) {
+                    const uint32_t x = d[(size_t)i * ld + j];
+                    if (x > (uint32_t)d[(size_t)j * ld + i]) v[e] = x;
+                }
+            }
+            cell[r][h] = as_u16x2(v[0] | (v[1] << 16));
+        }
+    // pivot k = 16 q + kk: its row lives at r = q in the lanes with ty == kk, its column in
+    // pair q / 2, half q % 2, in the lanes with tx == kk
+#pragma unroll
+    for (uint32_t q = 0; q < 8; ++q) {
+        for (uint32_t kk = 0; kk < 16; ++kk) {
+            const uint32_t k = 16 * q + kk;
+            if (k >= n) break;
+            const uint32_t buf = k & 1;
+            if (ty == kk) {
+#pragma unroll
+                for (uint32_t h = 0; h < 4; ++h) rowbuf[buf][16 * h + tx] = as_u32(cell[q][h]);
+            }
+            if (tx == kk) {
+#pragma unroll
+                for (uint32_t r = 0; r < 8; ++r) {
+                    const uint32_t v = (as_u32(cell[r][q >> 1]) >> (16 * (q & 1))) & 0xFFFFu;
+                    colbuf[buf][ty + 16 * r] = v | (v << 16);
+                }
+            }
+            __syncthreads();
+            u16x2 a[8], b[4];
+#pragma unroll
+            for (uint32_t r = 0; r < 8; ++r) a[r] = as_u16x2(colbuf[buf][ty + 16 * r]);
+#pragma unroll
+            for (uint32_t h = 0; h < 4; ++h) b[h] = as_u16x2(rowbuf[buf][16 * h + tx]);
+#pragma unroll
+            for (uint32_t r = 0; r < 8; ++r)
+#pragma unroll
+                for (uint32_t h = 0; h < 4; ++h) cell[r][h] = pk_max(cell[r][h], pk_min(a[r], b[h]));
+        }
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < 8; ++r)
+#pragma unroll
+        for (uint32_t h = 0; h < 4; ++h)
+#pragma unroll
+            for (uint32_t e = 0; e < 2; ++e) {
+                const uint32_t i = ty + 16 * r, j = tx + 32 * h + 16 * e;
+                const uint32_t v = (as_u32(cell[r][h]) >> (16 * e)) & 0xFFFFu;
+                if (i < n && j < n) out[(size_t)i * ldo + j] = (O)(i == j ? 0u : v);
+            }
+}
+
 // ---------------------------------------------------------------------------------------
 // Output: out[i][j] = W[i][j] widened to u32, 0 on the diagonal, padding dropped.
 
@@ -650,11 +721,11 @@ int closure_device_ex(DeviceContext* ctx, const void* d_any, bool d16, uint32_t 
         if (want_timing) (void)hipEventRecord(e0, s);
         // values of a 16-bit input fit the 16-bit output (p takes values of d only)
         if (d16 && out16) {
-            small_closure_kernel<uint16_t, uint16_t>
+            small_closure16_kernel<uint16_t>
                 <<<1, kThreads, 0, s>>>(d16p, n, ld, static_cast<uint16_t*>(out_any), ldo);
             *out16 = true;
         } else if (d16) {
-            small_closure_kernel<uint16_t, uint32_t>
+            small_closure16_kernel<uint32_t>
                 <<<1, kThreads, 0, s>>>(d16p, n, ld, static_cast<uint32_t*>(out_any), ldo);
         } else {
             small_closure_kernel<uint32_t, uint32_t>
